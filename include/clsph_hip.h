@@ -1,0 +1,150 @@
+/*
+ * clsph_hip: C ABI of the MI355X (gfx950) SPH substep pipeline.
+ *
+ * This is the layer that replaces the reference's OpenCL boundary
+ * (util/cl_boilerplate.{h,cpp} and the cl::CommandQueue calls of
+ * libclsph/sph_simulation.cpp). Plain pointers and sizes only; every call
+ * returns 0 on success or a negative CLSPH_E* code, with a message available
+ * from clsph_last_error(). Calls on one context are ordered on its stream.
+ *
+ * Ownership: the caller owns host arrays; the library owns device buffers.
+ * Threading: one context per host thread; a state belongs to one context.
+ */
+#ifndef CLSPH_HIP_H_
+#define CLSPH_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "clsph_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CLSPH_OK = 0,
+  CLSPH_EINVAL = -1,      /* bad argument */
+  CLSPH_EDEVICE = -2,     /* no usable GPU / HIP runtime error */
+  CLSPH_ENOMEM = -3,      /* device allocation failed */
+  CLSPH_EGRID = -4,       /* a grid dimension reached 1024 (sph_simulation.cpp:247-249) */
+  CLSPH_ETABLE = -5,      /* grid_cell_count exceeded the cell table capacity */
+};
+
+typedef struct clsph_ctx clsph_ctx;
+typedef struct clsph_state clsph_state;
+
+/* ---- context: replaces init_cl_single_device (util/cl_boilerplate.h:41,
+ * .cpp:3-63) and make_program (util/cl_boilerplate.h:42, .cpp:65-88); the
+ * kernels are compiled ahead of time for gfx950, so there is no runtime
+ * program build. `device` < 0 picks device 0, like the reference's
+ * "first platform, first device" default. */
+int clsph_init(int device, clsph_ctx** out);
+int clsph_free(clsph_ctx* ctx);
+/* Message of the last failing call on this context ("" if none). Replaces the
+ * check_cl_error print (util/cl_boilerplate.h:22-34). */
+const char* clsph_last_error(const clsph_ctx* ctx);
+/* Name of the device the context runs on. */
+int clsph_device_name(const clsph_ctx* ctx, char* buf, size_t len);
+/* Waits for the context's stream and reports kernel-raised errors. */
+int clsph_sync(clsph_ctx* ctx);
+
+/* ---- simulation state: replaces the front/back/sort_count cl::Buffers
+ * (sph_simulation.cpp:373-380). `table_cap` is the cell-table capacity in
+ * cells (0 = 2^30, enough for any 10-bit-per-axis grid). */
+int clsph_state_alloc(clsph_ctx* ctx, uint32_t n_cap, uint32_t table_cap,
+                      clsph_state** out);
+int clsph_state_free(clsph_state* st);
+
+/* Host parameters (the fields load_settings fills, sph_simulation.cpp:405-506);
+ * the grid fields are ignored and recomputed on the device each substep.
+ * `max_bounces` caps the collision loop (0 = 64). */
+int clsph_set_params(clsph_state* st, const clsph_simulation_parameters* params,
+                     const clsph_precomputed_kernel_values* terms,
+                     uint32_t max_bounces);
+
+/* Particle transfers in the reference's 80-byte layout. Upload replaces the
+ * write of sph_simulation.cpp:195-197 and computes the bounding box for the
+ * next substep; download replaces the read of :339-341 (particles come back in
+ * the last substep's sorted order with acceleration 0, as the reference's
+ * advection writes them). */
+int clsph_upload_aos(clsph_state* st, const clsph_particle* particles, uint32_t n);
+int clsph_download_aos(clsph_state* st, clsph_particle* particles, uint32_t n);
+
+/* Collision mesh: replaces the per-substep buffer creation and writes of
+ * sph_simulation.cpp:296-319 (uploaded once here). `indices` holds 3 vertex
+ * indices per face, `vertices` 3 floats per vertex, `face_normals` 3 floats
+ * per face. */
+int clsph_scene_upload(clsph_state* st, const float* face_normals,
+                       const float* vertices, uint32_t vertex_count,
+                       const uint32_t* indices, uint32_t face_count);
+
+/* ---- stages of one substep (simulate_single_frame, sph_simulation.cpp:173-344) */
+/* AABB + padding + grid dimensions: sph_simulation.cpp:199-252 */
+int clsph_grid(clsph_state* st);
+/* locate_in_grid: grid.cl:43-67, launched at sph_simulation.cpp:257-262 */
+int clsph_hash(clsph_state* st);
+/* sort_particles' radix passes (sph_simulation.cpp:107-160, sort.cl:23-59);
+ * also gathers the state into sorted order */
+int clsph_sort(clsph_state* st);
+/* cell table: sph_simulation.cpp:162-170, 270-276 */
+int clsph_cell_table(clsph_state* st);
+/* density_pressure: sph.cl:9-40, launched at sph_simulation.cpp:283-294 */
+int clsph_density_pressure(clsph_state* st);
+/* forces: sph.cl:42-62, launched at sph_simulation.cpp:322-327 */
+int clsph_forces(clsph_state* st);
+/* advection_collision: sph.cl:64-112, launched at sph_simulation.cpp:330-337 */
+int clsph_advect_collide(clsph_state* st);
+/* `nsteps` whole substeps (all stages above), replayed from one captured
+ * HIP graph; replaces simulate_single_frame's body. */
+int clsph_step(clsph_state* st, uint32_t nsteps);
+
+/* ---- readback of stage results (for checks and the save delegates) ---- */
+/* Device-computed grid fields of the current substep, written into *params. */
+int clsph_get_grid(clsph_state* st, clsph_simulation_parameters* params);
+/* Cell ids in canonical (pre-sort) order after clsph_hash. */
+int clsph_read_keys(clsph_state* st, uint32_t* keys, uint32_t n);
+/* Sorted cell ids and the permutation (sorted slot -> canonical index). */
+int clsph_read_sorted(clsph_state* st, uint32_t* keys, uint32_t* perm, uint32_t n);
+/* Cell table entries [0, count). */
+int clsph_read_cell_table(clsph_state* st, uint32_t* table, uint32_t count);
+/* The sorted particles as the reference's front/back buffer holds them after
+ * the last stage run: position, velocity, intermediate_velocity, acceleration
+ * (from forces), density, pressure, grid_index. */
+int clsph_download_sorted_aos(clsph_state* st, clsph_particle* particles, uint32_t n);
+/* Particles whose collision loop reached max_bounces in the last substep. */
+int clsph_bounce_capped(clsph_state* st, uint32_t* count);
+
+/* ---- timing on the context's stream (HIP events) ---- */
+/* Times `nsteps` graph-replayed substeps; writes elapsed milliseconds. */
+int clsph_time_steps(clsph_state* st, uint32_t nsteps, float* ms);
+/* Times one stage kernel sequence `reps` times in a row (stage: 0 grid,
+ * 1 hash, 2 sort, 3 cell table, 4 density, 5 forces, 6 advection); writes
+ * the average milliseconds per repetition. Stages are idempotent on the
+ * state except advection, which advances it. */
+int clsph_time_stage(clsph_state* st, int stage, uint32_t reps, float* ms);
+
+/* ---- scene builders (host only; no device needed) ---- */
+/* Dam-break block: a cubic lattice of `n` particles at `spacing` with
+ * round(extent/spacing) sites along x and z, stacked layer by layer from
+ * origin upward in y (x fastest, then z, then y) until n are placed; site
+ * (i, j, k) sits at origin + (i + 0.5, j + 0.5, k + 0.5) * spacing, each
+ * coordinate jittered by U(-jitter, +jitter) * spacing from a counter-based
+ * generator seeded by `seed` (extent[1] is not used). Velocities and all
+ * other fields are zero. Returns n. */
+uint32_t clsph_make_block(clsph_particle* out, uint32_t n, float spacing,
+                          const float origin[3], const float extent[3],
+                          float jitter, uint64_t seed);
+/* Smoothing-kernel constants for support radius h, with the reference's
+ * expressions (sph_simulation.cpp:499-505: float pow(h, 9.f) / pow(h, 6.f),
+ * double M_PI arithmetic, stored as float). */
+void clsph_kernel_terms(float h, clsph_precomputed_kernel_values* out);
+/* Axis-aligned box [lo, hi] as 12 triangles with inward unit normals:
+ * fills 8*3 vertex floats, 12*3 indices and 12*3 normal floats. */
+void clsph_make_box(const float lo[3], const float hi[3], float* vertices,
+                    uint32_t* indices, float* face_normals);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLSPH_HIP_H_ */

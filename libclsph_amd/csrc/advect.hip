@@ -1,0 +1,181 @@
+// Leapfrog advection with continuous collision against the scene triangles.
+//
+// Reference: advection_collision (sph.cl:64-112), advect (advection.cl:6-23),
+// detect_collision / respond / handle_collisions (collisions.cl:15-129).
+// Collision outcomes hinge on threshold tests (0 <= r <= 1, barycentric
+// s, t >= 0), so this kernel keeps the reference's FP32 operation order with
+// no contraction and IEEE division / square root. The reference's collision
+// do-while has no iteration cap; here it stops after max_bounces passes and
+// the particles that hit the cap are counted in DevParams::capped.
+//
+// The kernel also writes the per-block bounding box of the new positions, so
+// the next substep's grid parameters need no separate pass over the state.
+#include "state.h"
+
+namespace clsph {
+
+constexpr int kBlock = 256;
+
+struct V3 {
+  float x, y, z;
+};
+__device__ inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ inline V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ inline V3 operator*(float s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+__device__ inline V3 operator/(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+__device__ inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline float len(V3 a) { return sqrtf(dot(a, a)); }
+
+struct Mesh {
+  const float* __restrict__ normals;
+  const float* __restrict__ verts;
+  const uint32_t* __restrict__ idx;
+  uint32_t faces;
+};
+
+// detect_collision (collisions.cl:15-89): nearest hit along p0 -> p1; on equal
+// distance the later triangle wins.
+__device__ inline bool detect(V3 p0, V3 p1, const Mesh& m, V3& cpoint, V3& cnormal,
+                              float& depth) {
+#pragma clang fp contract(off)
+  bool hit = false;
+  V3 seg = p1 - p0;
+  float seg_len = len(seg);
+  for (uint32_t f = 0; f < m.faces; ++f) {
+    V3 nrm = v3(m.normals[3 * f + 0], m.normals[3 * f + 1], m.normals[3 * f + 2]);
+    if (dot(nrm, seg) / (len(nrm) * seg_len) <= 0.f) nrm = v3(-nrm.x, -nrm.y, -nrm.z);
+    const float* a = m.verts + 3 * m.idx[3 * f + 0];
+    const float* b = m.verts + 3 * m.idx[3 * f + 1];
+    const float* c = m.verts + 3 * m.idx[3 * f + 2];
+    V3 v0 = v3(a[0], a[1], a[2]);
+    V3 u = v3(b[0], b[1], b[2]) - v0;
+    V3 v = v3(c[0], c[1], c[2]) - v0;
+    float denom = dot(nrm, seg);
+    if (denom == 0.f) continue;
+    float r = dot(nrm, v0 - p0) / denom;
+    if (0.f <= r && r <= 1.f) {
+      V3 isect = p0 + r * seg;
+      V3 w = isect - v0;
+      float uv = dot(u, v), wv = dot(w, v), vv = dot(v, v), wu = dot(w, u), uu = dot(u, u);
+      float d2 = uv * uv - uu * vv;
+      float s = (uv * wv - vv * wu) / d2;
+      float t = (uv * wu - uu * wv) / d2;
+      if (s >= 0.f && t >= 0.f && s + t <= 1.f) {
+        if (hit && len(p0 - isect) > len(p0 - cpoint)) continue;
+        cnormal = nrm;
+        cpoint = isect;
+        depth = len(p1 - isect);
+        hit = true;
+      }
+    }
+  }
+  return hit;
+}
+
+__device__ inline float wave_min(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline float wave_max(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_advect(
+    const float4* __restrict__ pos_s, const float4* __restrict__ vh_s,
+    const float4* __restrict__ acc_s, Mesh mesh, DevParams* __restrict__ dp,
+    float4* __restrict__ pos, float4* __restrict__ vel, float4* __restrict__ vh,
+    float* __restrict__ aabb_partial, uint32_t n) {
+#pragma clang fp contract(off)
+  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  bool valid = i < n;
+  float4 outp = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t capped = 0;
+  if (valid) {
+    const float vmax = dp->p.max_velocity, e = dp->p.restitution;
+    const uint32_t max_b = dp->max_bounces;
+    float time_to_go = dp->p.time_delta * dp->p.simulation_scale;
+    float4 p4 = pos_s[i], vh4 = vh_s[i], a4 = acc_s[i];
+    V3 p = v3(p4.x, p4.y, p4.z), v = v3(vh4.x, vh4.y, vh4.z), a = v3(a4.x, a4.y, a4.z);
+    V3 rpos = p, rvel = v;
+    bool happened;
+    uint32_t it = 0;
+    do {
+      // advect (advection.cl:6-23)
+      V3 nv = v + a * time_to_go;
+      float nl = len(nv);
+      if (nl > vmax) nv = (nv / nl) * vmax;
+      V3 np = p + nv * time_to_go;
+      // handle_collisions (collisions.cl:109-129)
+      rpos = np;
+      rvel = nv;
+      float elapsed = time_to_go;
+      V3 cp = v3(0.f, 0.f, 0.f), cn = cp;
+      float depth = 0.f;
+      happened = detect(p, np, mesh, cp, cn, depth);
+      if (happened) {
+        // respond (collisions.cl:91-104)
+        rpos = cp - cn * 0.001f;
+        float k = (1.f + e * depth / (time_to_go * len(rvel))) * dot(rvel, cn);
+        rvel = rvel - k * cn;
+        elapsed = time_to_go * (len(rpos - p) / len(np - p));
+      }
+      p = rpos;
+      v = rvel;
+      time_to_go -= elapsed;
+      a = v3(0.f, 0.f, 0.f);
+      ++it;
+    } while (happened && it < max_b);
+    capped = happened ? 1u : 0u;
+    V3 vo = (v3(vh4.x, vh4.y, vh4.z) + rvel) / 2.f;
+    outp = make_float4(rpos.x, rpos.y, rpos.z, 0.f);
+    pos[i] = outp;
+    vel[i] = make_float4(vo.x, vo.y, vo.z, 0.f);
+    vh[i] = make_float4(rvel.x, rvel.y, rvel.z, 0.f);
+  }
+  // Per-block bounding box of the new positions (NaN skipped) and the count
+  // of particles that reached the bounce cap.
+  __shared__ float red[6][kBlock / 64];
+  __shared__ uint32_t cap_red[kBlock / 64];
+  const float inf = __builtin_huge_valf();
+  float bx[6];
+  bool vx = valid && outp.x == outp.x, vy = valid && outp.y == outp.y,
+       vz = valid && outp.z == outp.z;
+  bx[0] = wave_min(vx ? outp.x : inf);
+  bx[1] = wave_min(vy ? outp.y : inf);
+  bx[2] = wave_min(vz ? outp.z : inf);
+  bx[3] = wave_max(vx ? outp.x : -inf);
+  bx[4] = wave_max(vy ? outp.y : -inf);
+  bx[5] = wave_max(vz ? outp.z : -inf);
+  uint64_t cb = __builtin_amdgcn_ballot_w64(capped != 0);
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) {
+    for (int k = 0; k < 6; ++k) red[k][w] = bx[k];
+    cap_red[w] = (uint32_t)__builtin_popcountll(cb);
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    int k = threadIdx.x;
+    float r = red[k][0];
+    for (int q = 1; q < kBlock / 64; ++q) r = (k < 3) ? fminf(r, red[k][q]) : fmaxf(r, red[k][q]);
+    aabb_partial[blockIdx.x * 6 + k] = r;
+  }
+  if (threadIdx.x == 6) {
+    uint32_t c = 0;
+    for (int q = 0; q < kBlock / 64; ++q) c += cap_red[q];
+    if (c) atomicAdd(&dp->capped, c);
+  }
+}
+
+void launch_advect(clsph_state* s) {
+  if (s->n == 0) return;
+  Mesh m{s->face_normals, s->vertices, s->indices, s->face_count};
+  uint32_t blocks = (s->n + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_advect, dim3(blocks), dim3(kBlock), 0, s->ctx->stream, s->pos_s,
+                     s->vh_s, s->acc_s, m, s->dp, s->pos, s->vel, s->vh, s->aabb_partial,
+                     s->n);
+}
+
+}  // namespace clsph

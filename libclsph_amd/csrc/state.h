@@ -1,0 +1,89 @@
+// Host-side view of a simulation's device state and the stage launchers.
+#pragma once
+
+#include <string>
+
+#include "common.h"
+
+struct clsph_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  int cus = 256;  // compute units on the device
+};
+
+struct clsph_state {
+  clsph_ctx* ctx = nullptr;
+  uint32_t n_cap = 0;
+  uint32_t n = 0;
+  uint32_t table_cap = 0;
+
+  // Canonical particle arrays, in the order the reference's host array holds
+  // them at the start of a substep (the previous substep's sorted order).
+  float4* pos = nullptr;  // x, y, z, 0
+  float4* vel = nullptr;  // velocity
+  float4* vh = nullptr;   // intermediate_velocity
+
+  // Sorted working arrays for the current substep.
+  float4* pos_s = nullptr;
+  float4* vel_s = nullptr;
+  float4* vh_s = nullptr;
+  float4* acc_s = nullptr;  // acceleration from forces
+  float* rho_s = nullptr;   // density
+  float* prs_s = nullptr;   // pressure
+  float2* qm_s = nullptr;   // (p / rho^2, m / rho) per particle, for forces
+
+  // Radix sort buffers: keys/values ping-pong; after kSortPasses (odd) passes
+  // the sorted keys and permutation sit in key_b / val_b.
+  uint32_t* key_a = nullptr;
+  uint32_t* val_a = nullptr;
+  uint32_t* key_b = nullptr;
+  uint32_t* val_b = nullptr;
+  uint32_t* hist = nullptr;      // per-tile digit counts, digit-major
+  uint32_t* digit_base = nullptr;  // exclusive prefix of digit totals
+  uint32_t n_tiles_cap = 0;
+
+  uint32_t* table = nullptr;  // cell table, table_cap entries
+
+  // Per-block bounding boxes (6 floats per 256-particle block) written by the
+  // bounding-box kernel or by advection, reduced by the grid-parameter kernel.
+  float* aabb_partial = nullptr;
+
+  // Collision mesh (uploaded once, sph_simulation.cpp:296-319 re-uploads it
+  // every substep).
+  float* face_normals = nullptr;
+  float* vertices = nullptr;
+  uint32_t* indices = nullptr;
+  uint32_t face_count = 0;
+  uint32_t vert_floats = 0;
+
+  clsph::DevParams* dp = nullptr;  // device parameter block
+  clsph::DevParams host_dp{};      // host copy of the host-set fields
+
+  hipGraphExec_t graph = nullptr;  // one captured substep
+  bool graph_dirty = true;
+
+  // Last stage run (-1 after an upload) and whether the canonical
+  // acceleration is the zero vector advection leaves behind.
+  int stage = -1;
+  bool acc_zero = false;
+};
+
+namespace clsph {
+
+// grid.hip
+void launch_aabb(clsph_state* s);
+void launch_grid_params(clsph_state* s);
+void launch_hash(clsph_state* s);
+void launch_permute(clsph_state* s);
+void launch_cell_table(clsph_state* s);
+// sort.hip
+void launch_sort(clsph_state* s);
+uint32_t sort_tiles_for(uint32_t n);
+// sph.hip
+void launch_density(clsph_state* s);
+void launch_forces(clsph_state* s);
+// advect.hip
+void launch_advect(clsph_state* s);
+
+}  // namespace clsph

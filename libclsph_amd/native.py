@@ -1,0 +1,241 @@
+"""ctypes binding of the clsph HIP library (include/clsph_hip.h).
+
+The library is built in-tree (libclsph_amd/build.py) as libclsph_hip.so.
+Importing this module without it raises immediately: there is no CPU
+fallback for any entry point that computes.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import layout
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libclsph_hip.so")
+
+CLSPH_OK = 0
+ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "EGRID", -5: "ETABLE"}
+
+STAGES = {"grid": 0, "hash": 1, "sort": 2, "cell_table": 3, "density": 4, "forces": 5,
+          "advect": 6}
+
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_i32 = ctypes.c_int
+
+
+class ClsphError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"clsph error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m libclsph_amd.build` "
+            "(the HIP library is required; there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "clsph_init": (_i32, [_i32, ctypes.POINTER(_vp)]),
+        "clsph_free": (_i32, [_vp]),
+        "clsph_last_error": (ctypes.c_char_p, [_vp]),
+        "clsph_device_name": (_i32, [_vp, ctypes.c_char_p, ctypes.c_size_t]),
+        "clsph_sync": (_i32, [_vp]),
+        "clsph_state_alloc": (_i32, [_vp, _u32, _u32, ctypes.POINTER(_vp)]),
+        "clsph_state_free": (_i32, [_vp]),
+        "clsph_set_params": (_i32, [_vp, _vp, _vp, _u32]),
+        "clsph_upload_aos": (_i32, [_vp, _vp, _u32]),
+        "clsph_download_aos": (_i32, [_vp, _vp, _u32]),
+        "clsph_download_sorted_aos": (_i32, [_vp, _vp, _u32]),
+        "clsph_scene_upload": (_i32, [_vp, _vp, _vp, _u32, _vp, _u32]),
+        "clsph_grid": (_i32, [_vp]),
+        "clsph_hash": (_i32, [_vp]),
+        "clsph_sort": (_i32, [_vp]),
+        "clsph_cell_table": (_i32, [_vp]),
+        "clsph_density_pressure": (_i32, [_vp]),
+        "clsph_forces": (_i32, [_vp]),
+        "clsph_advect_collide": (_i32, [_vp]),
+        "clsph_step": (_i32, [_vp, _u32]),
+        "clsph_get_grid": (_i32, [_vp, _vp]),
+        "clsph_read_keys": (_i32, [_vp, _vp, _u32]),
+        "clsph_read_sorted": (_i32, [_vp, _vp, _vp, _u32]),
+        "clsph_read_cell_table": (_i32, [_vp, _vp, _u32]),
+        "clsph_bounce_capped": (_i32, [_vp, _vp]),
+        "clsph_time_steps": (_i32, [_vp, _u32, _vp]),
+        "clsph_time_stage": (_i32, [_vp, _i32, _u32, _vp]),
+        "clsph_make_block": (_u32, [_vp, _u32, ctypes.c_float, _vp, _vp, ctypes.c_float,
+                                    ctypes.c_uint64]),
+        "clsph_kernel_terms": (None, [ctypes.c_float, _vp]),
+        "clsph_make_box": (None, [_vp, _vp, _vp, _vp, _vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+# ---------------- host-only helpers (no device needed) ----------------
+
+def kernel_terms(h):
+    t = layout.terms()
+    lib.clsph_kernel_terms(ctypes.c_float(h), _ptr(t))
+    return t
+
+
+def make_block(n, spacing, origin, extent, jitter, seed):
+    out = layout.particles(n)
+    o = np.asarray(origin, dtype=np.float32)
+    e = np.asarray(extent, dtype=np.float32)
+    got = lib.clsph_make_block(_ptr(out), n, ctypes.c_float(spacing), _ptr(o), _ptr(e),
+                               ctypes.c_float(jitter), ctypes.c_uint64(seed))
+    return out[:got]
+
+
+def make_box(lo, hi):
+    lo = np.asarray(lo, dtype=np.float32)
+    hi = np.asarray(hi, dtype=np.float32)
+    verts = np.zeros(24, dtype=np.float32)
+    idx = np.zeros(36, dtype=np.uint32)
+    nrm = np.zeros(36, dtype=np.float32)
+    lib.clsph_make_box(_ptr(lo), _ptr(hi), _ptr(verts), _ptr(idx), _ptr(nrm))
+    return nrm, verts, idx
+
+
+# ---------------- device context / state ----------------
+
+class Context:
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        rc = lib.clsph_init(device, ctypes.byref(h))
+        if rc != CLSPH_OK:
+            raise ClsphError(rc, lib.clsph_last_error(None).decode())
+        self.h = h
+
+    def error(self):
+        return lib.clsph_last_error(self.h).decode()
+
+    def check(self, rc):
+        if rc != CLSPH_OK:
+            raise ClsphError(rc, self.error())
+
+    def device_name(self):
+        buf = ctypes.create_string_buffer(256)
+        self.check(lib.clsph_device_name(self.h, buf, 256))
+        return buf.value.decode()
+
+    def sync(self):
+        self.check(lib.clsph_sync(self.h))
+
+    def close(self):
+        if self.h:
+            lib.clsph_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class State:
+    """A device-resident particle system (one simulation)."""
+
+    def __init__(self, ctx, n_cap, table_cap=0):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(lib.clsph_state_alloc(ctx.h, n_cap, table_cap, ctypes.byref(h)))
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if self.h:
+            lib.clsph_state_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params, terms, max_bounces=0):
+        self.ctx.check(lib.clsph_set_params(self.h, _ptr(params), _ptr(terms), max_bounces))
+
+    def upload(self, parts):
+        parts = np.ascontiguousarray(parts, dtype=layout.PARTICLE)
+        self.ctx.check(lib.clsph_upload_aos(self.h, _ptr(parts), len(parts)))
+        self.n = len(parts)
+
+    def download(self):
+        out = layout.particles(self.n)
+        self.ctx.check(lib.clsph_download_aos(self.h, _ptr(out), self.n))
+        return out
+
+    def download_sorted(self):
+        out = layout.particles(self.n)
+        self.ctx.check(lib.clsph_download_sorted_aos(self.h, _ptr(out), self.n))
+        return out
+
+    def scene(self, normals, vertices, indices):
+        normals = np.ascontiguousarray(normals, dtype=np.float32)
+        vertices = np.ascontiguousarray(vertices, dtype=np.float32)
+        indices = np.ascontiguousarray(indices, dtype=np.uint32)
+        self.ctx.check(lib.clsph_scene_upload(self.h, _ptr(normals), _ptr(vertices),
+                                              len(vertices) // 3, _ptr(indices),
+                                              len(indices) // 3))
+
+    def stage(self, name):
+        fn = {"grid": lib.clsph_grid, "hash": lib.clsph_hash, "sort": lib.clsph_sort,
+              "cell_table": lib.clsph_cell_table, "density": lib.clsph_density_pressure,
+              "forces": lib.clsph_forces, "advect": lib.clsph_advect_collide}[name]
+        self.ctx.check(fn(self.h))
+
+    def step(self, nsteps=1):
+        self.ctx.check(lib.clsph_step(self.h, nsteps))
+
+    def grid(self):
+        p = layout.params()
+        self.ctx.check(lib.clsph_get_grid(self.h, _ptr(p)))
+        return p
+
+    def keys(self):
+        k = np.zeros(self.n, dtype=np.uint32)
+        self.ctx.check(lib.clsph_read_keys(self.h, _ptr(k), self.n))
+        return k
+
+    def sorted(self):
+        k = np.zeros(self.n, dtype=np.uint32)
+        p = np.zeros(self.n, dtype=np.uint32)
+        self.ctx.check(lib.clsph_read_sorted(self.h, _ptr(k), _ptr(p), self.n))
+        return k, p
+
+    def cell_table(self, count):
+        t = np.zeros(count, dtype=np.uint32)
+        self.ctx.check(lib.clsph_read_cell_table(self.h, _ptr(t), count))
+        return t
+
+    def bounce_capped(self):
+        c = np.zeros(1, dtype=np.uint32)
+        self.ctx.check(lib.clsph_bounce_capped(self.h, _ptr(c)))
+        return int(c[0])
+
+    def time_steps(self, nsteps):
+        ms = np.zeros(1, dtype=np.float32)
+        self.ctx.check(lib.clsph_time_steps(self.h, nsteps, _ptr(ms)))
+        return float(ms[0])
+
+    def time_stage(self, name, reps):
+        ms = np.zeros(1, dtype=np.float32)
+        self.ctx.check(lib.clsph_time_stage(self.h, STAGES[name], reps, _ptr(ms)))
+        return float(ms[0])
