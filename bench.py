@@ -1,0 +1,164 @@
+"""Throughput of the plain SPH substep on one MI355X.
+
+Workload: the 1,048,576-particle dam break (0.4 x 0.8 x 0.4 m block at
+spacing 0.5 h in a 1 m box, dt 1e-4 s). W untimed warm-up substeps, then K
+timed substeps replayed from one captured HIP graph, with state resident in
+HBM. Prints one JSON line with particle-steps/s, the roofline of the
+dominant kernel (timed live with HIP events on the library stream) and a CPU
+baseline (the oracle on the host cores over a bounded sample).
+
+    python bench.py [--gpus 1] [--steps 900] [--warmup 100]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per particle for each stage (FP32 SoA, each logical
+# array read or written once; positions/velocities count 12 B).
+#   hash     : x 12 read, key 4 + index 4 written
+#   sort     : 3 LSD passes x (key+index 8 read + 8 written), then the gather
+#              of x, v, v_half (index 4 + 36 read, 36 written)
+#   cell tab : sorted keys 4 read + 4 * G / N table written
+#   density  : x 12 read, rho and p 8 written
+#   forces   : x, v 24 + rho, p 8 read, a 12 written
+#   advect   : x, v_half, a 36 read, x, v, v_half 36 written
+SORT_PASSES = 3
+
+
+def stage_bytes(n, cells):
+    return {
+        "grid": 0.0,
+        "hash": 20.0 * n,
+        "sort": (16.0 * SORT_PASSES + 76.0) * n,
+        "cell_table": 4.0 * n + 4.0 * cells,
+        "density": 20.0 * n,
+        "forces": 44.0 * n,
+        "advect": 72.0 * n,
+    }
+
+
+def cpu_baseline(scene, budget_s=20.0, max_steps=3):
+    """The oracle (plain C port of the reference substep, OpenMP over the
+    host cores) on the same workload for a bounded number of substeps."""
+    import oracle
+    parts = scene.particles.copy()
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < max_steps:
+        parts, _, _, _ = oracle.run(parts, scene.params, scene.terms, scene.normals,
+                                    scene.vertices, scene.indices, 1)
+        steps += 1
+        if time.perf_counter() - t0 > budget_s / 2:
+            break
+    dt = time.perf_counter() - t0
+    return {
+        "value": scene.n * steps / dt,
+        "unit": "particle-steps/s",
+        "cores": oracle.threads(),
+        "kind": "port",
+        "sample": f"{steps} substep(s) of the {scene.n}-particle dam break from its "
+                  f"initial state, {dt:.1f} s on the host",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=900)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--particles", type=int, default=1048576)
+    ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the plain SPH substep runs on a single "
+              "MI355X in this build (multi-GPU sharding is a design, not a measured path)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from libclsph_amd import native, scenes
+
+    n = args.particles
+    scene = scenes.dam_break(n)
+    ctx = native.Context(0)
+    st = native.State(ctx, n)
+    st.set_params(scene.params, scene.terms)
+    st.scene(scene.normals, scene.vertices, scene.indices)
+    st.upload(scene.particles)
+    print(f"# device {ctx.device_name()}; {n} particles", file=sys.stderr, flush=True)
+
+    if args.warmup:
+        st.step(args.warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ms = st.time_steps(args.steps)  # HIP events around K graph replays
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = ms / args.steps
+    value = n * args.steps / (ms / 1e3)
+    print(f"# timed {args.steps} substeps: {ms:.2f} ms (events), {wall * 1e3:.2f} ms (wall)",
+          file=sys.stderr, flush=True)
+
+    # Per-stage durations on the same state (HIP events between stages).
+    prof = st.profile_steps(args.profile_steps)
+    grid = st.grid()
+    cells = int(grid["grid_cell_count"])
+    nbytes = stage_bytes(n, cells)
+    dom = max((k for k in prof if k != "grid"), key=lambda k: prof[k])
+    achieved = nbytes[dom] / (prof[dom] * 1e-3) / 1e9
+    stages = {k: {"ms": round(v, 5),
+                  "GB/s": round(nbytes[k] / (v * 1e-3) / 1e9, 1) if v > 0 else None}
+              for k, v in prof.items()}
+    print("# stages " + json.dumps(stages), file=sys.stderr, flush=True)
+
+    final = st.download()
+    finite = bool(np.isfinite(final["position"][:, :3]).all())
+    capped = st.bounce_capped()
+
+    out = {
+        "metric": "particle-steps/s @1M particles (plain SPH substep)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "dam_break",
+            "particles": n,
+            "dt": float(scene.params["time_delta"]),
+            "timed_substeps": f"{args.warmup + 1}-{args.warmup + args.steps}",
+            "grid_cell_count_end": cells,
+            "state_finite": finite,
+            "bounce_capped_last": capped,
+        },
+        "stages_ms": {k: v["ms"] for k, v in stages.items()},
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(scene)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -124,6 +124,12 @@ int clsph_time_steps(clsph_state* st, uint32_t nsteps, float* ms);
  * state except advection, which advances it. */
 int clsph_time_stage(clsph_state* st, int stage, uint32_t reps, float* ms);
 
+/* Runs `nsteps` whole substeps stage by stage with HIP events recorded
+ * between stages on the context stream; writes the average milliseconds
+ * per substep of each stage into stage_ms[0..6] (stage order as in
+ * clsph_time_stage; stage 2 includes the state gather into sorted order). */
+int clsph_profile_steps(clsph_state* st, uint32_t nsteps, float* stage_ms);
+
 /* ---- scene builders (host only; no device needed) ---- */
 /* Dam-break block: a cubic lattice of `n` particles at `spacing` with
  * round(extent/spacing) sites along x and z, stacked layer by layer from

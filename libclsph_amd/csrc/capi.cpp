@@ -68,7 +68,8 @@ void free_state_buffers(clsph_state* st) {
   void* bufs[] = {st->pos,  st->vel,   st->vh,    st->pos_s, st->vel_s, st->vh_s,
                   st->acc_s, st->rho_s, st->prs_s, st->qm_s,  st->key_a, st->val_a,
                   st->key_b, st->val_b, st->hist,  st->digit_base, st->table,
-                  st->aabb_partial, st->face_normals, st->vertices, st->indices, st->dp};
+                  st->aabb_partial, st->face_normals, st->vertices, st->indices, st->dp,
+                  st->nbr, st->nbr_count};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (st->graph) (void)hipGraphExecDestroy(st->graph);
@@ -196,6 +197,8 @@ int clsph_state_alloc(clsph_ctx* ctx, uint32_t n_cap, uint32_t table_cap, clsph_
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->rho_s, n_cap);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->prs_s, n_cap);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->qm_s, n_cap);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->nbr, (size_t)kMaxNeighbours * n_cap);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->nbr_count, n_cap);
   uint32_t** u4[] = {&st->key_a, &st->val_a, &st->key_b, &st->val_b};
   for (auto p : u4)
     if (rc == CLSPH_OK) rc = dalloc(ctx, p, n_cap);
@@ -485,6 +488,35 @@ int clsph_time_stage(clsph_state* st, int stage, uint32_t reps, float* ms) {
   *ms = t / (float)reps;
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
+  return device_errors(st);
+}
+
+int clsph_profile_steps(clsph_state* st, uint32_t nsteps, float* stage_ms) {
+  if (!st || !stage_ms) return CLSPH_EINVAL;
+  constexpr int kStages = 7;
+  std::vector<hipEvent_t> ev((size_t)nsteps * (kStages + 1));
+  for (auto& e : ev) HIPCK(st->ctx, hipEventCreate(&e));
+  for (uint32_t k = 0; k < nsteps; ++k) {
+    hipEvent_t* e = ev.data() + (size_t)k * (kStages + 1);
+    HIPCK(st->ctx, hipEventRecord(e[0], st->ctx->stream));
+    for (int s = 0; s < kStages; ++s) {
+      int rc = run_stage(st, s);
+      if (rc) return rc;
+      HIPCK(st->ctx, hipEventRecord(e[s + 1], st->ctx->stream));
+    }
+  }
+  HIPCK(st->ctx, hipEventSynchronize(ev.back()));
+  for (int s = 0; s < kStages; ++s) stage_ms[s] = 0.f;
+  for (uint32_t k = 0; k < nsteps; ++k) {
+    hipEvent_t* e = ev.data() + (size_t)k * (kStages + 1);
+    for (int s = 0; s < kStages; ++s) {
+      float t = 0.f;
+      HIPCK(st->ctx, hipEventElapsedTime(&t, e[s], e[s + 1]));
+      stage_ms[s] += t;
+    }
+  }
+  for (int s = 0; s < kStages; ++s) stage_ms[s] /= (float)(nsteps ? nsteps : 1);
+  for (auto& e : ev) (void)hipEventDestroy(e);
   return device_errors(st);
 }
 

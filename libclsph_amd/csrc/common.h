@@ -19,6 +19,9 @@ enum : uint32_t {
 constexpr int kSortPasses = 3;       // LSD passes; digit width adapts to bitlen(G)
 constexpr int kMaxDigitBits = 10;    // 3 x 10 bits covers any 30-bit Morton key
 constexpr int kMaxBins = 1 << kMaxDigitBits;
+// Neighbour-list slots per particle; a particle with more neighbours is
+// handled by forces with a cell walk instead.
+constexpr uint32_t kMaxNeighbours = 128;
 
 // Device-resident parameter block. `p` mirrors the reference's
 // simulation_parameters; its grid fields (min/max point, grid sizes,

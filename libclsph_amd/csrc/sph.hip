@@ -56,12 +56,18 @@ __device__ inline uint32_t usable_cells(const DevParams* __restrict__ dp) {
   return G > dp->table_cap ? 0u : G;
 }
 
+// Besides density and pressure, records the particle's neighbours (r < h,
+// self excluded) in visiting order into nbr[k * n + i], k < kMaxNeighbours,
+// with the full count in nbr_count[i]; forces walks this list.
 __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ pos,
                                                     const uint32_t* __restrict__ table,
                                                     const DevParams* __restrict__ dp,
                                                     float* __restrict__ rho,
                                                     float* __restrict__ prs,
-                                                    float2* __restrict__ qm, uint32_t n) {
+                                                    float2* __restrict__ qm,
+                                                    uint32_t* __restrict__ nbr,
+                                                    uint32_t* __restrict__ nbr_count,
+                                                    uint32_t n) {
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const float h = dp->p.h, h2 = h * h;
@@ -69,6 +75,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ p
   float4 xi = pos[i];
   CellWalk w = octant_block(xi, dp);
   float acc = 0.f;
+  uint32_t cnt = 0;
   for (uint32_t dz = 0; dz < 2; ++dz)
     for (uint32_t dy = 0; dy < 2; ++dy)
       for (uint32_t dx0 = 0; dx0 < 2; ++dx0) {
@@ -78,10 +85,17 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ p
           float4 xj = pos[j];
           float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
           float d2 = dx * dx + dy * dy + dz * dz;
-          float t = h2 - d2;
-          acc += (d2 < h2) ? t * t * t : 0.f;
+          if (d2 < h2) {
+            float t = h2 - d2;
+            acc += t * t * t;
+            if (j != i) {
+              if (cnt < kMaxNeighbours) nbr[(size_t)cnt * n + i] = j;
+              ++cnt;
+            }
+          }
         }
       }
+  nbr_count[i] = cnt;
   // density = sum m * poly6 * (h^2 - r^2)^3 ; Tait pressure (sph.cl:36-39)
   float m = dp->p.particle_mass;
   float d = m * (dp->t.poly_6 * acc);
@@ -93,11 +107,53 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ p
   qm[i] = make_float2(p / (d * d), m / d);
 }
 
+// Pair terms of compute_internal_forces_with_grid (forces.cl:60-101) for one
+// neighbour j != i at squared distance d2 < h^2.
+struct ForceSums {
+  float3 pt, vt, nrm;
+  float lap;
+};
+
+__device__ inline void force_pair(ForceSums& f, float dx, float dy, float dz, float d2,
+                                  float4 vi, float4 vj, float2 qmi, float2 qmj, float h, float h2,
+                                  float m, float c_spiky, float c_visc, float c_grad,
+                                  float c_lap, float spiky0) {
+  float r = __builtin_amdgcn_sqrtf(d2);
+  float t = h2 - d2;
+  float hr = h - r;
+  // pressure (forces.cl:66-73): (p_j/rho_j^2 + p_i/rho_i^2) m grad W_spiky
+  float ps = (qmj.x + qmi.x) * m;
+  if (r < 1e-7f) {
+    float g = ps * spiky0;
+    f.pt.x += g;
+    f.pt.y += g;
+    f.pt.z += g;
+  } else {
+    float g = ps * c_spiky * hr * hr * __builtin_amdgcn_rcpf(r);
+    f.pt.x += g * dx;
+    f.pt.y += g * dy;
+    f.pt.z += g * dz;
+  }
+  // viscosity (forces.cl:75-80): (v_j - v_i) (m/rho_j) lap W_visc
+  float vw = qmj.y * c_visc * hr;
+  f.vt.x += (vj.x - vi.x) * vw;
+  f.vt.y += (vj.y - vi.y) * vw;
+  f.vt.z += (vj.z - vi.z) * vw;
+  // colour field gradient and laplacian (forces.cl:83-91)
+  float gw = qmj.y * c_grad * t * t;
+  f.nrm.x += gw * dx;
+  f.nrm.y += gw * dy;
+  f.nrm.z += gw * dz;
+  f.lap += qmj.y * c_lap * t * (3.f * h2 - 7.f * d2);
+}
+
 __global__ __launch_bounds__(kBlock) void k_forces(const float4* __restrict__ pos,
                                                    const float4* __restrict__ vel,
                                                    const float* __restrict__ rho,
                                                    const float2* __restrict__ qm,
                                                    const uint32_t* __restrict__ table,
+                                                   const uint32_t* __restrict__ nbr,
+                                                   const uint32_t* __restrict__ nbr_count,
                                                    const DevParams* __restrict__ dp,
                                                    float4* __restrict__ acc_out, uint32_t n) {
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -112,52 +168,43 @@ __global__ __launch_bounds__(kBlock) void k_forces(const float4* __restrict__ po
   float4 xi = pos[i], vi = vel[i];
   float2 qmi = qm[i];
   float rho_i = rho[i];
-  float3 pt = make_float3(0.f, 0.f, 0.f), vt = pt, nrm = pt;
-  float lap = 0.f;
-  CellWalk w = octant_block(xi, dp);
-  for (uint32_t dz = 0; dz < 2; ++dz)
-    for (uint32_t dy = 0; dy < 2; ++dy)
-      for (uint32_t dx0 = 0; dx0 < 2; ++dx0) {
-        uint32_t s, e;
-        cell_range(morton3(w.c0[0] + dx0, w.c0[1] + dy, w.c0[2] + dz), table, G, n, s, e);
-        for (uint32_t j = s; j < e; ++j) {
-          float4 xj = pos[j];
-          float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-          float d2 = dx * dx + dy * dy + dz * dz;
-          if (!(d2 < h2)) continue;
-          float2 qmj = qm[j];
-          float r = __builtin_amdgcn_sqrtf(d2);
-          float t = h2 - d2;
-          if (j != i) {
-            float hr = h - r;
-            // pressure (forces.cl:66-73): (p_j/rho_j^2 + p_i/rho_i^2) m grad W_spiky
-            float ps = (qmj.x + qmi.x) * m;
-            if (r < 1e-7f) {
-              float g = ps * spiky0;
-              pt.x += g;
-              pt.y += g;
-              pt.z += g;
-            } else {
-              float g = ps * c_spiky * hr * hr * __builtin_amdgcn_rcpf(r);
-              pt.x += g * dx;
-              pt.y += g * dy;
-              pt.z += g * dz;
-            }
-            // viscosity (forces.cl:75-80): (v_j - v_i) (m/rho_j) lap W_visc
-            float4 vj = vel[j];
-            float vw = qmj.y * c_visc * hr;
-            vt.x += (vj.x - vi.x) * vw;
-            vt.y += (vj.y - vi.y) * vw;
-            vt.z += (vj.z - vi.z) * vw;
+  ForceSums fs;
+  fs.pt = make_float3(0.f, 0.f, 0.f);
+  fs.vt = fs.pt;
+  fs.nrm = fs.pt;
+  // Self term of the colour-field laplacian: (m/rho_i) poly6_lap h^2 (3 h^2).
+  fs.lap = qmi.y * c_lap * h2 * (3.f * h2);
+  uint32_t cnt = nbr_count[i];
+  if (cnt <= kMaxNeighbours) {
+    for (uint32_t k = 0; k < cnt; ++k) {
+      uint32_t j = nbr[(size_t)k * n + i];
+      float4 xj = pos[j], vj = vel[j];
+      float2 qmj = qm[j];
+      float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+      float d2 = dx * dx + dy * dy + dz * dz;
+      force_pair(fs, dx, dy, dz, d2, vi, vj, qmi, qmj, h, h2, m, c_spiky, c_visc, c_grad, c_lap,
+                 spiky0);
+    }
+  } else {
+    // More neighbours than the list holds: walk the cells again.
+    CellWalk w = octant_block(xi, dp);
+    for (uint32_t dz = 0; dz < 2; ++dz)
+      for (uint32_t dy = 0; dy < 2; ++dy)
+        for (uint32_t dx0 = 0; dx0 < 2; ++dx0) {
+          uint32_t s, e;
+          cell_range(morton3(w.c0[0] + dx0, w.c0[1] + dy, w.c0[2] + dz), table, G, n, s, e);
+          for (uint32_t j = s; j < e; ++j) {
+            float4 xj = pos[j];
+            float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            float d2 = dx * dx + dy * dy + dz * dz;
+            if (!(d2 < h2) || j == i) continue;
+            force_pair(fs, dx, dy, dz, d2, vi, vel[j], qmi, qm[j], h, h2, m, c_spiky, c_visc,
+                       c_grad, c_lap, spiky0);
           }
-          // colour field gradient and laplacian (forces.cl:83-91), self included
-          float gw = qmj.y * c_grad * t * t;
-          nrm.x += gw * dx;
-          nrm.y += gw * dy;
-          nrm.z += gw * dz;
-          lap += qmj.y * c_lap * t * (3.f * h2 - 7.f * d2);
         }
-      }
+  }
+  float3 pt = fs.pt, vt = fs.vt, nrm = fs.nrm;
+  float lap = fs.lap;
   // forces.cl:95-111 and sph.cl:53-58
   float mu = dp->p.dynamic_viscosity;
   float3 f = make_float3(-rho_i * pt.x + vt.x * mu, -rho_i * pt.y + vt.y * mu,
@@ -178,13 +225,15 @@ static inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock
 void launch_density(clsph_state* s) {
   if (s->n == 0) return;
   hipLaunchKernelGGL(k_density, dim3(blocks_for(s->n)), dim3(kBlock), 0, s->ctx->stream,
-                     s->pos_s, s->table, s->dp, s->rho_s, s->prs_s, s->qm_s, s->n);
+                     s->pos_s, s->table, s->dp, s->rho_s, s->prs_s, s->qm_s, s->nbr, s->nbr_count,
+                     s->n);
 }
 
 void launch_forces(clsph_state* s) {
   if (s->n == 0) return;
   hipLaunchKernelGGL(k_forces, dim3(blocks_for(s->n)), dim3(kBlock), 0, s->ctx->stream,
-                     s->pos_s, s->vel_s, s->rho_s, s->qm_s, s->table, s->dp, s->acc_s, s->n);
+                     s->pos_s, s->vel_s, s->rho_s, s->qm_s, s->table, s->nbr, s->nbr_count, s->dp,
+                     s->acc_s, s->n);
 }
 
 }  // namespace clsph

@@ -32,6 +32,8 @@ struct clsph_state {
   float* rho_s = nullptr;   // density
   float* prs_s = nullptr;   // pressure
   float2* qm_s = nullptr;   // (p / rho^2, m / rho) per particle, for forces
+  uint32_t* nbr = nullptr;        // neighbour lists, slot-major: nbr[k * n + i]
+  uint32_t* nbr_count = nullptr;  // neighbours per particle
 
   // Radix sort buffers: keys/values ping-pong; after kSortPasses (odd) passes
   // the sorted keys and permutation sit in key_b / val_b.

@@ -65,6 +65,7 @@ def _load():
         "clsph_bounce_capped": (_i32, [_vp, _vp]),
         "clsph_time_steps": (_i32, [_vp, _u32, _vp]),
         "clsph_time_stage": (_i32, [_vp, _i32, _u32, _vp]),
+        "clsph_profile_steps": (_i32, [_vp, _u32, _vp]),
         "clsph_make_block": (_u32, [_vp, _u32, ctypes.c_float, _vp, _vp, ctypes.c_float,
                                     ctypes.c_uint64]),
         "clsph_kernel_terms": (None, [ctypes.c_float, _vp]),
@@ -234,6 +235,12 @@ class State:
         ms = np.zeros(1, dtype=np.float32)
         self.ctx.check(lib.clsph_time_steps(self.h, nsteps, _ptr(ms)))
         return float(ms[0])
+
+    def profile_steps(self, nsteps):
+        """Average ms per substep of each stage, keyed by stage name."""
+        ms = np.zeros(7, dtype=np.float32)
+        self.ctx.check(lib.clsph_profile_steps(self.h, nsteps, _ptr(ms)))
+        return {name: float(ms[i]) for name, i in STAGES.items()}
 
     def time_stage(self, name, reps):
         ms = np.zeros(1, dtype=np.float32)

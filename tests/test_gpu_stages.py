@@ -27,13 +27,13 @@ def ctx():
     c.close()
 
 
-def cloud(n, seed, dup=0):
-    """Random particles in [0.2, 0.8]^3 with `dup` exact duplicates appended
+def cloud(n, seed, dup=0, lo=0.2, hi=0.8):
+    """Random particles in [lo, hi]^3 with `dup` exact duplicates appended
     (coincident particles exercise the spiky special case and equal keys)."""
     rng = np.random.default_rng(seed)
     sc = scenes.dam_break(max(n, 1))
     parts = sc.particles.copy()[:n]
-    parts["position"][:, :3] = rng.uniform(0.2, 0.8, size=(n, 3)).astype(np.float32)
+    parts["position"][:, :3] = rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)
     parts["velocity"][:, :3] = rng.normal(0, 0.05, size=(n, 3)).astype(np.float32)
     parts["intermediate_velocity"][:, :3] = rng.normal(0, 0.05, size=(n, 3)).astype(np.float32)
     if dup:
@@ -81,6 +81,8 @@ CASES = {
     "cloud63": lambda: cloud(63, 3),
     "cloud1000dup": lambda: cloud(1000, 4, dup=50),
     "cloud4097": lambda: cloud(4097, 5),
+    # 300 particles inside one h-sphere: more neighbours than a list holds
+    "dense300": lambda: cloud(300, 6, lo=0.49, hi=0.51),
 }
 
 
