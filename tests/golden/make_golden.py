@@ -1,0 +1,64 @@
+"""Writes the golden fixtures under tests/golden/ from the CPU oracle.
+
+  dam4096_step0.npz  : the 4,096-particle dam break (seed 1234): initial
+                       positions, then every stage output of one substep
+                       (grid fields, cell ids, stable order, cell table,
+                       density, pressure, acceleration, advected state)
+  dam16k_series.npz  : the 16,384-particle dam break run 1,000 substeps:
+                       mean density and kinetic energy after each substep
+
+    python tests/golden/make_golden.py [--skip-series]
+"""
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from libclsph_amd import scenes  # noqa: E402
+
+
+def step0():
+    sc = scenes.dam_break(4096, seed=1234)
+    r = oracle.stages(sc.particles, sc.params, sc.terms, sc.normals, sc.vertices, sc.indices)
+    p = r.params
+    np.savez_compressed(
+        os.path.join(HERE, "dam4096_step0.npz"),
+        position0=sc.particles["position"][:, :3],
+        min_point=p["min_point"][:3], max_point=p["max_point"][:3],
+        grid_size=np.array([p["grid_size_x"], p["grid_size_y"], p["grid_size_z"]]),
+        grid_cell_count=np.array(p["grid_cell_count"]),
+        keys=r.hashed["grid_index"],
+        order=np.argsort(r.hashed["grid_index"], kind="stable").astype(np.uint32),
+        table=r.table,
+        density=r.density["density"], pressure=r.density["pressure"],
+        acceleration=r.forces["acceleration"][:, :3],
+        position1=r.advected["position"][:, :3],
+        velocity1=r.advected["velocity"][:, :3],
+        intermediate_velocity1=r.advected["intermediate_velocity"][:, :3],
+    )
+
+
+def series(steps=1000):
+    sc = scenes.dam_break(16384)
+    t0 = time.time()
+    _, rho, ke, _ = oracle.run(sc.particles, sc.params, sc.terms, sc.normals, sc.vertices,
+                               sc.indices, steps)
+    print(f"series: {steps} substeps in {time.time() - t0:.0f} s")
+    np.savez_compressed(os.path.join(HERE, "dam16k_series.npz"), mean_density=rho,
+                        kinetic_energy=ke)
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--skip-series", action="store_true")
+    a = ap.parse_args()
+    step0()
+    if not a.skip_series:
+        series()
