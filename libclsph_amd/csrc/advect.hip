@@ -10,6 +10,8 @@
 //
 // The kernel also writes the per-block bounding box of the new positions, so
 // the next substep's grid parameters need no separate pass over the state.
+// Results go to each particle's slot in the reference's order (stable by
+// cell id), which is the order the next substep starts from.
 #include "state.h"
 
 namespace clsph {
@@ -85,9 +87,9 @@ __device__ inline float wave_max(float v) {
 
 __global__ __launch_bounds__(kBlock) void k_advect(
     const float4* __restrict__ pos_s, const float4* __restrict__ vh_s,
-    const float4* __restrict__ acc_s, Mesh mesh, DevParams* __restrict__ dp,
-    float4* __restrict__ pos, float4* __restrict__ vel, float4* __restrict__ vh,
-    float* __restrict__ aabb_partial, uint32_t n) {
+    const float4* __restrict__ acc_s, const uint32_t* __restrict__ slot, Mesh mesh,
+    DevParams* __restrict__ dp, float4* __restrict__ pos, float4* __restrict__ vel,
+    float4* __restrict__ vh, float* __restrict__ aabb_partial, uint32_t n) {
 #pragma clang fp contract(off)
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   bool valid = i < n;
@@ -131,9 +133,12 @@ __global__ __launch_bounds__(kBlock) void k_advect(
     capped = happened ? 1u : 0u;
     V3 vo = (v3(vh4.x, vh4.y, vh4.z) + rvel) / 2.f;
     outp = make_float4(rpos.x, rpos.y, rpos.z, 0.f);
-    pos[i] = outp;
-    vel[i] = make_float4(vo.x, vo.y, vo.z, 0.f);
-    vh[i] = make_float4(rvel.x, rvel.y, rvel.z, 0.f);
+    uint32_t o = slot[i];
+    if (o < n) {  // a valid permutation always passes; this guards misuse
+      pos[o] = outp;
+      vel[o] = make_float4(vo.x, vo.y, vo.z, 0.f);
+      vh[o] = make_float4(rvel.x, rvel.y, rvel.z, 0.f);
+    }
   }
   // Per-block bounding box of the new positions (NaN skipped) and the count
   // of particles that reached the bounce cap.
@@ -174,8 +179,8 @@ void launch_advect(clsph_state* s) {
   Mesh m{s->face_normals, s->vertices, s->indices, s->face_count};
   uint32_t blocks = (s->n + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_advect, dim3(blocks), dim3(kBlock), 0, s->ctx->stream, s->pos_s,
-                     s->vh_s, s->acc_s, m, s->dp, s->pos, s->vel, s->vh, s->aabb_partial,
-                     s->n);
+                     s->vh_s, s->acc_s, s->slot, m, s->dp, s->pos, s->vel, s->vh,
+                     s->aabb_partial, s->n);
 }
 
 }  // namespace clsph

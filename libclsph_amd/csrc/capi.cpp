@@ -66,10 +66,10 @@ int device_errors(clsph_state* st) {
 
 void free_state_buffers(clsph_state* st) {
   void* bufs[] = {st->pos,  st->vel,   st->vh,    st->pos_s, st->vel_s, st->vh_s,
-                  st->acc_s, st->rho_s, st->prs_s, st->qm_s,  st->key_a, st->val_a,
+                  st->acc_s, st->rho_s, st->prs_s, st->fa, st->fb, st->key_a, st->val_a,
                   st->key_b, st->val_b, st->hist,  st->digit_base, st->table,
                   st->aabb_partial, st->face_normals, st->vertices, st->indices, st->dp,
-                  st->nbr, st->nbr_count};
+                  st->nbr, st->nbr_count, st->slot};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (st->graph) (void)hipGraphExecDestroy(st->graph);
@@ -96,8 +96,12 @@ int run_stage(clsph_state* st, int stage) {
       launch_hash(st);
       break;
     case 2:
+      // sorting also builds the tables and the canonical slots, so that a
+      // slot is always current once particles have been reordered
       launch_sort(st);
       launch_permute(st);
+      launch_cell_table(st);
+      st->fields_canonical = false;
       break;
     case 3:
       launch_cell_table(st);
@@ -182,7 +186,12 @@ int clsph_sync(clsph_ctx* ctx) {
 int clsph_state_alloc(clsph_ctx* ctx, uint32_t n_cap, uint32_t table_cap, clsph_state** out) {
   if (!ctx || !out) return fail(ctx, CLSPH_EINVAL, "clsph_state_alloc: NULL argument");
   *out = nullptr;
-  if (table_cap == 0) table_cap = 1u << 30;
+  if (table_cap == 0) {
+    uint64_t c = 8ull * n_cap;
+    table_cap = (uint32_t)(c < (1u << 20) ? (1u << 20) : (c > (1u << 28) ? (1u << 28) : c));
+  }
+  if (table_cap > (1u << 29))
+    return fail(ctx, CLSPH_EINVAL, "table_cap above 2^29 cells is not supported");
   HIPCK(ctx, hipSetDevice(ctx->device));
   clsph_state* st = new clsph_state();
   st->ctx = ctx;
@@ -196,15 +205,17 @@ int clsph_state_alloc(clsph_ctx* ctx, uint32_t n_cap, uint32_t table_cap, clsph_
     if (rc == CLSPH_OK) rc = dalloc(ctx, p, n_cap);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->rho_s, n_cap);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->prs_s, n_cap);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->qm_s, n_cap);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->nbr, (size_t)kMaxNeighbours * n_cap);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->fa, n_cap);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->fb, n_cap);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->nbr, (size_t)(kMaxNeighbours / 4) * n_cap);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->nbr_count, n_cap);
   uint32_t** u4[] = {&st->key_a, &st->val_a, &st->key_b, &st->val_b};
   for (auto p : u4)
     if (rc == CLSPH_OK) rc = dalloc(ctx, p, n_cap);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->hist, (size_t)kMaxBins * st->n_tiles_cap);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->digit_base, 2 * (size_t)kMaxBins);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->table, table_cap);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->table, 8ull * table_cap + 1);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->slot, n_cap);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->aabb_partial, (size_t)nb * 6);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->dp, 1);
   if (rc != CLSPH_OK) {
@@ -281,6 +292,7 @@ int clsph_upload_aos(clsph_state* st, const clsph_particle* p, uint32_t n) {
   HIPCK(st->ctx, hipMemcpyAsync(&st->dp->n, &st->host_dp.n, sizeof(uint32_t), hipMemcpyHostToDevice, s));
   launch_aabb(st);
   st->acc_zero = false;
+  st->fields_canonical = true;
   st->stage = -1;
   int rc = check_launch(st->ctx);
   if (rc) return rc;
@@ -291,9 +303,12 @@ int clsph_upload_aos(clsph_state* st, const clsph_particle* p, uint32_t n) {
 static int download_common(clsph_state* st, clsph_particle* out, uint32_t n, bool sorted) {
   if (!st || (!out && n)) return fail(st ? st->ctx : nullptr, CLSPH_EINVAL, "NULL argument");
   if (n > st->n) return fail(st->ctx, CLSPH_EINVAL, "asked for more particles than are held");
+  if (n == 0) return device_errors(st);
+  if (n != st->n && !st->fields_canonical)
+    return fail(st->ctx, CLSPH_EINVAL, "partial downloads need the whole particle count");
   std::vector<float4> pos(n), vel(n), vh(n), acc(n);
   std::vector<float> rho(n), prs(n);
-  std::vector<uint32_t> key(n);
+  std::vector<uint32_t> key(n), slot(n);
   hipStream_t s = st->ctx->stream;
   const float4* P = sorted ? st->pos_s : st->pos;
   const float4* V = sorted ? st->vel_s : st->vel;
@@ -307,19 +322,31 @@ static int download_common(clsph_state* st, clsph_particle* out, uint32_t n, boo
   HIPCK(st->ctx, hipMemcpyAsync(rho.data(), st->rho_s, sizeof(float) * n, hipMemcpyDeviceToHost, s));
   HIPCK(st->ctx, hipMemcpyAsync(prs.data(), st->prs_s, sizeof(float) * n, hipMemcpyDeviceToHost, s));
   HIPCK(st->ctx, hipMemcpyAsync(key.data(), st->key_b, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+  bool reorder = !st->fields_canonical;
+  if (reorder)
+    HIPCK(st->ctx, hipMemcpyAsync(slot.data(), st->slot, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
   HIPCK(st->ctx, hipStreamSynchronize(s));
-  for (uint32_t i = 0; i < n; ++i) {
-    clsph_particle& q = out[i];
+  // Fields computed in half-cell order go to each particle's canonical slot;
+  // the sorted-order arrays themselves are also in half-cell order.
+  for (uint32_t k = 0; k < n; ++k) {
+    uint32_t o = reorder ? slot[k] : k;
+    if (o >= n) return fail(st->ctx, CLSPH_EDEVICE, "canonical slots are not a permutation");
+    clsph_particle& q = out[o];
     memset(&q, 0, sizeof(q));
-    q.position.s[0] = pos[i].x, q.position.s[1] = pos[i].y, q.position.s[2] = pos[i].z;
-    q.velocity.s[0] = vel[i].x, q.velocity.s[1] = vel[i].y, q.velocity.s[2] = vel[i].z;
-    q.intermediate_velocity.s[0] = vh[i].x, q.intermediate_velocity.s[1] = vh[i].y,
-    q.intermediate_velocity.s[2] = vh[i].z;
-    if (want_acc)
-      q.acceleration.s[0] = acc[i].x, q.acceleration.s[1] = acc[i].y, q.acceleration.s[2] = acc[i].z;
-    q.density = rho[i];
-    q.pressure = prs[i];
-    q.grid_index = key[i];
+    q.density = rho[k];
+    q.pressure = prs[k];
+    q.grid_index = reorder ? key[k] >> 3 : key[k];
+    if (want_acc && (sorted || !reorder)) {
+      q.acceleration.s[0] = acc[k].x, q.acceleration.s[1] = acc[k].y, q.acceleration.s[2] = acc[k].z;
+    }
+  }
+  for (uint32_t k = 0; k < n; ++k) {
+    uint32_t o = (sorted && reorder) ? slot[k] : k;
+    clsph_particle& q = out[o];
+    q.position.s[0] = pos[k].x, q.position.s[1] = pos[k].y, q.position.s[2] = pos[k].z;
+    q.velocity.s[0] = vel[k].x, q.velocity.s[1] = vel[k].y, q.velocity.s[2] = vel[k].z;
+    q.intermediate_velocity.s[0] = vh[k].x, q.intermediate_velocity.s[1] = vh[k].y,
+    q.intermediate_velocity.s[2] = vh[k].z;
   }
   return device_errors(st);
 }
@@ -404,6 +431,7 @@ int clsph_step(clsph_state* st, uint32_t nsteps) {
   if (rc) return rc;
   for (uint32_t k = 0; k < nsteps; ++k) HIPCK(st->ctx, hipGraphLaunch(st->graph, st->ctx->stream));
   st->acc_zero = true;
+  st->fields_canonical = false;
   st->stage = 6;
   return CLSPH_OK;
 }
@@ -429,22 +457,38 @@ static int read_u32(clsph_state* st, const uint32_t* src, uint32_t* dst, size_t 
 int clsph_read_keys(clsph_state* st, uint32_t* keys, uint32_t n) {
   if (!st) return CLSPH_EINVAL;
   if (n > st->n) return fail(st->ctx, CLSPH_EINVAL, "n exceeds the particle count");
-  return read_u32(st, st->key_a, keys, n);
+  int rc = read_u32(st, st->key_a, keys, n);
+  for (uint32_t i = 0; !rc && i < n; ++i) keys[i] >>= 3;  // half-cell key -> cell id
+  return rc;
 }
 
 int clsph_read_sorted(clsph_state* st, uint32_t* keys, uint32_t* perm, uint32_t n) {
   if (!st) return CLSPH_EINVAL;
-  if (n > st->n) return fail(st->ctx, CLSPH_EINVAL, "n exceeds the particle count");
-  int rc = keys ? read_u32(st, st->key_b, keys, n) : CLSPH_OK;
-  if (!rc && perm) rc = read_u32(st, st->val_b, perm, n);
-  return rc;
+  if (n != st->n) return fail(st->ctx, CLSPH_EINVAL, "n must be the particle count");
+  std::vector<uint32_t> k(n), v(n), slot(n);
+  int rc = read_u32(st, st->key_b, k.data(), n);
+  if (!rc) rc = read_u32(st, st->val_b, v.data(), n);
+  if (!rc) rc = read_u32(st, st->slot, slot.data(), n);
+  if (rc) return rc;
+  // Half-cell order -> the reference's order (stable by cell id).
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t o = slot[i];
+    if (o >= n) return fail(st->ctx, CLSPH_EDEVICE, "canonical slots are not a permutation");
+    if (keys) keys[o] = k[i] >> 3;
+    if (perm) perm[o] = v[i];
+  }
+  return CLSPH_OK;
 }
 
 int clsph_read_cell_table(clsph_state* st, uint32_t* table, uint32_t count) {
   if (!st) return CLSPH_EINVAL;
   if (count > st->table_cap) return fail(st->ctx, CLSPH_EINVAL, "count exceeds table capacity");
-  int rc = read_u32(st, st->table, table, count);
-  return rc ? rc : device_errors(st);
+  if (count == 0) return device_errors(st);
+  std::vector<uint32_t> ht(8ull * count);
+  int rc = read_u32(st, st->table, ht.data(), ht.size());
+  if (rc) return rc;
+  for (uint32_t c = 0; c < count; ++c) table[c] = ht[8ull * c];  // T[c] = ht[8c]
+  return device_errors(st);
 }
 
 int clsph_bounce_capped(clsph_state* st, uint32_t* count) {
@@ -467,6 +511,7 @@ int clsph_time_steps(clsph_state* st, uint32_t nsteps, float* ms) {
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   st->acc_zero = true;
+  st->fields_canonical = false;
   st->stage = 6;
   return device_errors(st);
 }

@@ -13,11 +13,15 @@ namespace clsph {
 enum : uint32_t {
   kErrGridDim = 1u,        // a grid dimension reached 1024 (sph_simulation.cpp:247-249)
   kErrTableCap = 2u,       // grid_cell_count exceeds the cell table allocation
-  kErrSortBits = 4u,       // key width exceeds what the configured passes cover
+  kErrFineKey = 4u,        // half-cell keys need more than 32 bits (grid_cell_count >= 2^29)
 };
 
-constexpr int kSortPasses = 3;       // LSD passes; digit width adapts to bitlen(G)
-constexpr int kMaxDigitBits = 10;    // 3 x 10 bits covers any 30-bit Morton key
+// Particles are sorted by the Morton key of the half-size cell (side h) they
+// fall in. Its top bits are the reference's cell id (side 2h), so the order
+// groups each cell's particles by octant; the reference's stable order by
+// cell id is recovered from it per particle.
+constexpr int kSortPasses = 3;       // LSD passes; digit width adapts to the key width
+constexpr int kMaxDigitBits = 11;    // 3 x 11 bits covers any 32-bit key
 constexpr int kMaxBins = 1 << kMaxDigitBits;
 // Neighbour-list slots per particle; a particle with more neighbours is
 // handled by forces with a cell walk instead.
@@ -30,8 +34,8 @@ struct DevParams {
   clsph_simulation_parameters p;   // 128 B
   clsph_precomputed_kernel_values t;
   uint32_t n;            // particle count
-  uint32_t table_cap;    // cell table capacity (entries)
-  uint32_t key_bits;     // bitlen(grid_cell_count)
+  uint32_t table_cap;    // cell capacity of the tables (half-cell table holds 8x + 1)
+  uint32_t key_bits;     // width of the half-cell keys: bitlen(grid_cell_count) + 3
   uint32_t digit_bits;   // per LSD pass
   uint32_t err;          // kErr* bits, sticky until cleared by the host
   uint32_t capped;       // particles whose collision loop hit max_bounces (last step)

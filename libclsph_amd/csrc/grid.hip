@@ -1,9 +1,16 @@
 // Grid stages: bounding box, grid parameters, cell-id hashing, state
-// permutation and the cell-start table.
+// permutation, cell tables and the reference's stable order.
 //
 // The bounding box / grid arithmetic must reproduce the reference's host FP32
 // sequence bit for bit (sph_simulation.cpp:201-252), and the cell id the
 // kernel's (grid.cl:43-67): no contraction, correctly rounded division.
+//
+// Particles are hashed to the half-size cell (side h) they fall in. With
+// u = (x - min) / (2h) per axis, the cell coordinate of the reference is
+// (uint)u and the half-cell coordinate is (uint)(2u) = 2 (uint)u + [frac(u) >=
+// 1/2], exact because 2u is exact. The Morton key of the half-cell is then
+// (reference key << 3) | octant, so sorting by it keeps the reference's cells
+// contiguous and in order, each grouped by octant.
 #include "state.h"
 
 namespace clsph {
@@ -105,15 +112,17 @@ __global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dp
   uint32_t G = morton3(gs[0], gs[1], gs[2]);
   dp->p.grid_cell_count = G;
   if (G > dp->table_cap) err |= kErrTableCap;
-  uint32_t kb = bitlen(G);
+  uint32_t kb = bitlen(G) + 3;
+  if (kb > 32) err |= kErrFineKey;
   dp->key_bits = kb;
   uint32_t db = (kb + kSortPasses - 1) / kSortPasses;
-  dp->digit_bits = db < 1 ? 1 : db;
+  dp->digit_bits = db > (uint32_t)kMaxDigitBits ? (uint32_t)kMaxDigitBits : db;
   dp->capped = 0;
   if (err) atomicOr(&dp->err, err);
 }
 
-// locate_in_grid (grid.cl:43-67): key = Morton((uint)((x - min) / (h * 2))).
+// locate_in_grid (grid.cl:43-67): the reference's cell id is
+// Morton((uint)((x - min) / (h * 2))); the half-cell key adds the octant.
 __global__ __launch_bounds__(kBlock) void k_hash(const float4* __restrict__ pos,
                                                  const DevParams* __restrict__ dp,
                                                  uint32_t* __restrict__ key,
@@ -124,10 +133,14 @@ __global__ __launch_bounds__(kBlock) void k_hash(const float4* __restrict__ pos,
   if (i >= n) return;
   float4 p = pos[i];
   float L = dp->p.h * 2;
-  uint32_t cx = (uint32_t)((p.x - dp->p.min_point.s[0]) / L);
-  uint32_t cy = (uint32_t)((p.y - dp->p.min_point.s[1]) / L);
-  uint32_t cz = (uint32_t)((p.z - dp->p.min_point.s[2]) / L);
-  key[i] = morton3(cx, cy, cz);
+  float u[3] = {(p.x - dp->p.min_point.s[0]) / L, (p.y - dp->p.min_point.s[1]) / L,
+                (p.z - dp->p.min_point.s[2]) / L};
+  uint32_t c[3], oct = 0;
+  for (int a = 0; a < 3; ++a) {
+    c[a] = (uint32_t)u[a];
+    oct |= (((uint32_t)(2.f * u[a])) & 1u) << a;
+  }
+  key[i] = (morton3(c[0], c[1], c[2]) << 3) | oct;
   val[i] = i;
 }
 
@@ -145,25 +158,84 @@ __global__ __launch_bounds__(kBlock) void k_permute(
   vh_s[i] = vh[j];
 }
 
-// Cell-start table T[c] = #{sorted keys < c} for c < grid_cell_count
-// (sph_simulation.cpp:162-170): lower_bound over the sorted keys.
+// First index in [lo, hi) whose key is >= k (keys sorted ascending).
+__device__ inline uint32_t lower_bound(const uint32_t* __restrict__ keys, uint32_t lo,
+                                       uint32_t hi, uint32_t k) {
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (keys[mid] < k)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Half-cell table: ht[f] = #{keys < f} for f in [0, 8G], one thread per cell
+// of side 2h. The reference's cell table (sph_simulation.cpp:162-170),
+// T[c] = #{cell ids < c}, is ht[8c].
 __global__ __launch_bounds__(kBlock) void k_cell_table(const uint32_t* __restrict__ keys,
                                                        uint32_t n,
                                                        const DevParams* __restrict__ dp,
-                                                       uint32_t* __restrict__ table) {
+                                                       uint32_t* __restrict__ ht) {
   uint32_t G = dp->p.grid_cell_count;
-  if (G > dp->table_cap) G = dp->table_cap;
+  if (G > dp->table_cap || (dp->err & kErrFineKey)) return;
   for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < G; c += gridDim.x * kBlock) {
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-      uint32_t mid = (lo + hi) >> 1;
-      if (keys[mid] < c)
-        lo = mid + 1;
-      else
-        hi = mid;
+    uint32_t start = lower_bound(keys, 0, n, c << 3);
+    // end of the cell: gallop from start, then bisect
+    uint32_t nxt = (c + 1) << 3;
+    uint32_t lo = start, step = 64, hi = start;
+    while (hi < n && keys[hi] < nxt) {
+      lo = hi + 1;
+      hi = (n - hi > step) ? hi + step : n;
+      step <<= 1;
     }
-    table[c] = lo;
+    uint32_t end = lower_bound(keys, lo, hi, nxt);
+    ht[8u * c] = start;
+    uint32_t b = start;
+    for (uint32_t o = 1; o < 8; ++o) {
+      b = lower_bound(keys, b, end, (c << 3) | o);
+      ht[8u * c + o] = b;
+    }
+    if (c == G - 1) ht[8u * G] = n;
   }
+}
+
+// Slot of each sorted particle in the reference's order (stable by cell id).
+// Within a cell the half-cell order is octant-major with the canonical
+// (input) order kept inside each octant, so the particle's rank in its cell
+// is its index in its octant plus the members of the other octants that came
+// before it in the input.
+__global__ __launch_bounds__(kBlock) void k_canonical_slot(const uint32_t* __restrict__ keys,
+                                                           const uint32_t* __restrict__ perm,
+                                                           const uint32_t* __restrict__ ht,
+                                                           const DevParams* __restrict__ dp,
+                                                           uint32_t* __restrict__ slot,
+                                                           uint32_t n) {
+  uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n) return;
+  uint32_t G = dp->p.grid_cell_count;
+  if (G > dp->table_cap || (dp->err & kErrFineKey)) {
+    slot[k] = k;
+    return;
+  }
+  uint32_t f = keys[k], c = f >> 3, o = f & 7u;
+  uint32_t me = perm[k];
+  uint32_t rank = k - ht[f];
+  for (uint32_t q = 0; q < 8; ++q) {
+    if (q == o) continue;
+    uint32_t lo = ht[8u * c + q], hi = ht[8u * c + q + 1];
+    uint32_t a = lo, b = hi;
+    while (a < b) {
+      uint32_t mid = (a + b) >> 1;
+      if (perm[mid] < me)
+        a = mid + 1;
+      else
+        b = mid;
+    }
+    rank += a - lo;
+  }
+  slot[k] = ht[8u * c] + rank;
 }
 
 static inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
@@ -192,9 +264,12 @@ void launch_permute(clsph_state* s) {
 }
 
 void launch_cell_table(clsph_state* s) {
+  if (s->n == 0) return;
   uint32_t blocks = (uint32_t)s->ctx->cus * 8;
   hipLaunchKernelGGL(k_cell_table, dim3(blocks), dim3(kBlock), 0, s->ctx->stream,
                      s->key_b, s->n, s->dp, s->table);
+  hipLaunchKernelGGL(k_canonical_slot, dim3(blocks_for(s->n)), dim3(kBlock), 0,
+                     s->ctx->stream, s->key_b, s->val_b, s->table, s->dp, s->slot, s->n);
 }
 
 }  // namespace clsph

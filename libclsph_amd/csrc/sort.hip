@@ -1,12 +1,14 @@
-// Stable LSD radix sort of (cell id, particle index) pairs.
+// Stable LSD radix sort of (key, particle index) pairs.
 //
 // The reference sorts whole 80-byte particles with 4 passes of 8-bit digits
 // over 128 work-items and a host scan (sort.cl:23-59,
-// sph_simulation.cpp:107-156); its result is the stable order by grid_index.
-// Here the pairs are sorted in kSortPasses passes whose digit width is chosen
-// on the device from bitlen(grid_cell_count), so a captured step needs no
-// host round trip. Each pass is count -> scan -> scatter over 1024-key tiles;
-// the scatter ranks keys stably inside a tile with wave64 ballot matching.
+// sph_simulation.cpp:107-156); its result is the stable order by key. Here
+// the pairs are sorted in kSortPasses passes whose digit width is chosen on
+// the device from the key width, so a captured step needs no host round
+// trip. Each pass: per-tile digit counts laid out digit-major, an exclusive
+// scan of each digit's counts over the tiles plus the digit totals (together
+// the same prefix as the reference's host scan over that layout), and a
+// scatter that ranks keys stably inside a tile with wave64 ballot matching.
 #include "state.h"
 
 namespace clsph {
@@ -22,16 +24,27 @@ __device__ inline uint32_t digit_of(uint32_t key, uint32_t shift, uint32_t mask)
   return (key >> shift) & mask;
 }
 
+// Digit width of this sort, clamped to max_db (the width the launch sized its
+// shared memory for; a wider one only arises with the table-overflow error set).
+__device__ inline void pass_digit(const DevParams* __restrict__ dp, uint32_t max_db, int pass,
+                                  uint32_t& nb, uint32_t& mask, uint32_t& shift) {
+  uint32_t db = dp->digit_bits;
+  db = db > max_db ? max_db : db;
+  nb = 1u << db;
+  mask = nb - 1;
+  shift = db * (uint32_t)pass;
+}
+
 // Per-tile digit histogram, stored digit-major: hist[d * n_tiles + tile].
 __global__ __launch_bounds__(kSortBlock) void k_sort_count(const uint32_t* __restrict__ keys,
                                                            uint32_t n,
                                                            const DevParams* __restrict__ dp,
-                                                           int pass,
+                                                           uint32_t max_db, int pass,
                                                            uint32_t* __restrict__ hist,
                                                            uint32_t n_tiles) {
-  __shared__ uint32_t bins[kMaxBins];
-  uint32_t db = dp->digit_bits;
-  uint32_t nb = 1u << db, mask = nb - 1, shift = db * (uint32_t)pass;
+  extern __shared__ uint32_t bins[];  // 1 << max_db entries
+  uint32_t nb, mask, shift;
+  pass_digit(dp, max_db, pass, nb, mask, shift);
   for (uint32_t d = threadIdx.x; d < nb; d += kSortBlock) bins[d] = 0;
   __syncthreads();
   uint32_t base = blockIdx.x * kTile;
@@ -45,10 +58,10 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_count(const uint32_t* __res
     hist[d * n_tiles + blockIdx.x] = bins[d];
 }
 
-// Block-wide exclusive scan of one value per thread; returns the prefix and
-// writes the block total to *total.
-__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
-  __shared__ uint32_t wsum[kSortBlock / 64];
+// Block-wide exclusive scan of one value per thread (blockDim.x = B).
+template <int B>
+__device__ inline uint32_t block_exclusive_scan(uint32_t v) {
+  __shared__ uint32_t wsum[B / 64];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint32_t x = v;
   for (int o = 1; o < 64; o <<= 1) {
@@ -57,60 +70,35 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
   }
   if (lane == 63) wsum[w] = x;
   __syncthreads();
-  uint32_t off = 0, tot = 0;
-  for (int q = 0; q < kSortBlock / 64; ++q) {
-    if (q < w) off += wsum[q];
-    tot += wsum[q];
-  }
+  uint32_t off = 0;
+  for (int q = 0; q < w; ++q) off += wsum[q];
   __syncthreads();
-  *total = tot;
   return off + x - v;
 }
 
-// One block per digit: exclusive scan over tiles in place, digit total out.
-__global__ __launch_bounds__(kSortBlock) void k_sort_scan_tiles(uint32_t* __restrict__ hist,
-                                                                uint32_t n_tiles,
-                                                                const DevParams* __restrict__ dp,
-                                                                uint32_t* __restrict__ totals) {
+// One block per digit: exclusive scan of that digit's per-tile counts, in
+// place, and the digit's total.
+__global__ __launch_bounds__(kSortBlock) void k_sort_scan(uint32_t* __restrict__ hist,
+                                                          uint32_t n_tiles,
+                                                          const DevParams* __restrict__ dp,
+                                                          uint32_t max_db,
+                                                          uint32_t* __restrict__ totals) {
   uint32_t d = blockIdx.x;
-  if (d >= (1u << dp->digit_bits)) return;
+  uint32_t nb, mask, shift;
+  pass_digit(dp, max_db, 0, nb, mask, shift);
+  if (d >= nb) return;
   uint32_t* row = hist + (size_t)d * n_tiles;
   uint32_t per = (n_tiles + kSortBlock - 1) / kSortBlock;
   uint32_t s = threadIdx.x * per, e = min(s + per, n_tiles);
   uint32_t local = 0;
   for (uint32_t k = s; k < e; ++k) local += row[k];
-  uint32_t tot;
-  uint32_t run = block_exclusive_scan(local, &tot);
+  uint32_t run = block_exclusive_scan<kSortBlock>(local);
   for (uint32_t k = s; k < e; ++k) {
     uint32_t c = row[k];
     row[k] = run;
     run += c;
   }
-  if (threadIdx.x == 0) totals[d] = tot;
-}
-
-// One block: exclusive scan of the digit totals (at most kMaxBins).
-__global__ __launch_bounds__(kSortBlock) void k_sort_scan_digits(const uint32_t* __restrict__ totals,
-                                                                 const DevParams* __restrict__ dp,
-                                                                 uint32_t* __restrict__ digit_base) {
-  uint32_t nb = 1u << dp->digit_bits;
-  constexpr int per = kMaxBins / kSortBlock;
-  uint32_t v[per];
-  uint32_t local = 0;
-#pragma unroll
-  for (int k = 0; k < per; ++k) {
-    uint32_t d = threadIdx.x * per + k;
-    v[k] = d < nb ? totals[d] : 0u;
-    local += v[k];
-  }
-  uint32_t tot;
-  uint32_t run = block_exclusive_scan(local, &tot);
-#pragma unroll
-  for (int k = 0; k < per; ++k) {
-    uint32_t d = threadIdx.x * per + k;
-    if (d < nb) digit_base[d] = run;
-    run += v[k];
-  }
+  if (threadIdx.x == kSortBlock - 1) totals[d] = run;
 }
 
 // Stable scatter. Wave w of the tile owns keys [w*256, w*256+256) of the tile,
@@ -120,14 +108,39 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scan_digits(const uint32_t*
 __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n,
-    const DevParams* __restrict__ dp, int pass, const uint32_t* __restrict__ hist,
-    const uint32_t* __restrict__ digit_base, uint32_t n_tiles) {
-  __shared__ uint32_t wcnt[kWaves][kMaxBins];
-  uint32_t db = dp->digit_bits;
-  uint32_t nb = 1u << db, mask = nb - 1, shift = db * (uint32_t)pass;
+    const DevParams* __restrict__ dp, uint32_t max_db, int pass,
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ totals,
+    uint32_t n_tiles) {
+  // Shared memory: kWaves running counts per digit, then the digit starts.
+  extern __shared__ uint32_t smem[];
+  const uint32_t nbmax = 1u << max_db;
+  uint32_t* wcnt = smem;                         // wcnt[q * nbmax + d]
+  uint32_t* digit_start = smem + kWaves * nbmax;
+  uint32_t nb, mask, shift;
+  pass_digit(dp, max_db, pass, nb, mask, shift);
+  uint32_t db = 31u - (uint32_t)__builtin_clz(nb);
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (uint32_t d = threadIdx.x; d < nb; d += kSortBlock)
-    for (int q = 0; q < kWaves; ++q) wcnt[q][d] = 0;
+    for (int q = 0; q < kWaves; ++q) wcnt[q * nbmax + d] = 0;
+  // Exclusive scan of the digit totals: where each digit's run starts.
+  {
+    constexpr int per = kMaxBins / kSortBlock;
+    uint32_t v[per];
+    uint32_t local = 0;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+      uint32_t d = threadIdx.x * per + k;
+      v[k] = d < nb ? totals[d] : 0u;
+      local += v[k];
+    }
+    uint32_t run = block_exclusive_scan<kSortBlock>(local);
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+      uint32_t d = threadIdx.x * per + k;
+      if (d < nb) digit_start[d] = run;
+      run += v[k];
+    }
+  }
   __syncthreads();
 
   uint32_t kk[kSortItems], vv[kSortItems], dd[kSortItems], rr[kSortItems];
@@ -152,7 +165,7 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(
     // running count and hands the old value to its peers.
     uint32_t leader = m ? (uint32_t)__builtin_ctzll(m) : 0u;
     uint32_t prev = 0;
-    if (valid && peer == 0) prev = atomicAdd(&wcnt[w][dig], cnt);
+    if (valid && peer == 0) prev = atomicAdd(&wcnt[w * nbmax + dig], cnt);
     prev = __shfl(prev, (int)leader, 64);
     kk[t] = key;
     vv[t] = val;
@@ -164,8 +177,8 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(
   for (uint32_t d = threadIdx.x; d < nb; d += kSortBlock) {
     uint32_t run = 0;
     for (int q = 0; q < kWaves; ++q) {
-      uint32_t c = wcnt[q][d];
-      wcnt[q][d] = run;
+      uint32_t c = wcnt[q * nbmax + d];
+      wcnt[q * nbmax + d] = run;
       run += c;
     }
   }
@@ -174,7 +187,8 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(
   for (int t = 0; t < kSortItems; ++t) {
     uint32_t dig = dd[t];
     if (dig == 0xFFFFFFFFu) continue;
-    uint32_t p = digit_base[dig] + hist[(size_t)dig * n_tiles + blockIdx.x] + wcnt[w][dig] + rr[t];
+    uint32_t p = digit_start[dig] + offsets[(size_t)dig * n_tiles + blockIdx.x] +
+                 wcnt[w * nbmax + dig] + rr[t];
     keys_out[p] = kk[t];
     vals_out[p] = vv[t];
   }
@@ -188,16 +202,21 @@ void launch_sort(clsph_state* s) {
   uint32_t* vin = s->val_a;
   uint32_t* kout = s->key_b;
   uint32_t* vout = s->val_b;
-  uint32_t* totals = s->digit_base + kMaxBins;
+  // Widest digit the state can need: keys stay below 8 * table_cap while the
+  // cell count fits the tables.
+  uint32_t kb = bitlen(s->table_cap) + 3;
+  uint32_t max_db = (kb + kSortPasses - 1) / kSortPasses;
+  max_db = max_db > (uint32_t)kMaxDigitBits ? (uint32_t)kMaxDigitBits : max_db;
+  uint32_t nbmax = 1u << max_db;
+  size_t count_lds = sizeof(uint32_t) * nbmax;
+  size_t scatter_lds = sizeof(uint32_t) * nbmax * (kWaves + 1);
   for (int pass = 0; pass < kSortPasses; ++pass) {
-    hipLaunchKernelGGL(k_sort_count, dim3(nt), dim3(kSortBlock), 0, st, kin, s->n, s->dp,
-                       pass, s->hist, nt);
-    hipLaunchKernelGGL(k_sort_scan_tiles, dim3(kMaxBins), dim3(kSortBlock), 0, st, s->hist,
-                       nt, s->dp, totals);
-    hipLaunchKernelGGL(k_sort_scan_digits, dim3(1), dim3(kSortBlock), 0, st, totals, s->dp,
-                       s->digit_base);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(nt), dim3(kSortBlock), 0, st, kin, vin, kout,
-                       vout, s->n, s->dp, pass, s->hist, s->digit_base, nt);
+    hipLaunchKernelGGL(k_sort_count, dim3(nt), dim3(kSortBlock), count_lds, st, kin, s->n,
+                       s->dp, max_db, pass, s->hist, nt);
+    hipLaunchKernelGGL(k_sort_scan, dim3(nbmax), dim3(kSortBlock), 0, st, s->hist, nt, s->dp,
+                       max_db, s->digit_base);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(nt), dim3(kSortBlock), scatter_lds, st, kin, vin,
+                       kout, vout, s->n, s->dp, max_db, pass, s->hist, s->digit_base, nt);
     uint32_t* tk = kin;
     kin = kout;
     kout = tk;

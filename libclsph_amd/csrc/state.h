@@ -31,8 +31,9 @@ struct clsph_state {
   float4* acc_s = nullptr;  // acceleration from forces
   float* rho_s = nullptr;   // density
   float* prs_s = nullptr;   // pressure
-  float2* qm_s = nullptr;   // (p / rho^2, m / rho) per particle, for forces
-  uint32_t* nbr = nullptr;        // neighbour lists, slot-major: nbr[k * n + i]
+  float4* fa = nullptr;     // force inputs: (x, y, z, p / rho^2)
+  float4* fb = nullptr;     // force inputs: (vx, vy, vz, m / rho)
+  uint4* nbr = nullptr;           // neighbour lists, 4 per entry: nbr[q * n + i]
   uint32_t* nbr_count = nullptr;  // neighbours per particle
 
   // Radix sort buffers: keys/values ping-pong; after kSortPasses (odd) passes
@@ -45,7 +46,11 @@ struct clsph_state {
   uint32_t* digit_base = nullptr;  // exclusive prefix of digit totals
   uint32_t n_tiles_cap = 0;
 
-  uint32_t* table = nullptr;  // cell table, table_cap entries
+  // Half-cell table (8 * table_cap + 1 entries): table[f] = #{keys < f}; the
+  // reference's cell table is table[8c]. slot[k]: the canonical position
+  // (stable order by cell id) of the particle at sorted position k.
+  uint32_t* table = nullptr;
+  uint32_t* slot = nullptr;
 
   // Per-block bounding boxes (6 floats per 256-particle block) written by the
   // bounding-box kernel or by advection, reduced by the grid-parameter kernel.
@@ -69,6 +74,9 @@ struct clsph_state {
   // acceleration is the zero vector advection leaves behind.
   int stage = -1;
   bool acc_zero = false;
+  // True while rho_s / prs_s / key_b / acc_s still hold the uploaded
+  // particles in canonical order (no substep has run since the upload).
+  bool fields_canonical = true;
 };
 
 namespace clsph {
