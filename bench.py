@@ -1,13 +1,17 @@
 """Throughput of the plain SPH substep on one MI355X.
 
-Workload: the 1,048,576-particle dam break (0.4 x 0.8 x 0.4 m block at
-spacing 0.5 h in a 1 m box, dt 1e-4 s). W untimed warm-up substeps, then K
-timed substeps replayed from one captured HIP graph, with state resident in
-HBM. Prints one JSON line with particle-steps/s, the roofline of the
+Default workload: the 1,048,576-particle dam break (0.4 x 0.8 x 0.4 m block
+at spacing 0.5 h in a 1 m box, dt 1e-4 s). W untimed warm-up substeps, then
+K timed substeps replayed from one captured HIP graph, with state resident
+in HBM. Prints one JSON line with particle-steps/s, the roofline of the
 dominant kernel (timed live with HIP events on the library stream) and a CPU
 baseline (the oracle on the host cores over a bounded sample).
 
-    python bench.py [--gpus 1] [--steps 900] [--warmup 100]
+--workload heightfield: 4,194,304 particles poured from a random-velocity
+column onto the 128 x 128 heightfield (32,768 triangles + 10 wall
+triangles), dt 5e-5 s, 50 + 450 substeps.
+
+    python bench.py [--gpus 1] [--steps 900] [--warmup 100] [--workload dam_break]
 """
 import argparse
 import json
@@ -43,15 +47,18 @@ def stage_bytes(n, cells):
     }
 
 
-def cpu_baseline(scene, budget_s=20.0, max_steps=3):
+def cpu_baseline(scene, label, budget_s=20.0, max_steps=3):
     """The oracle (plain C port of the reference substep, OpenMP over the
-    host cores) on the same workload for a bounded number of substeps."""
+    host cores, collisions tested against every triangle as the reference
+    does) on a bounded number of substeps of `scene`."""
     import oracle
     parts = scene.particles.copy()
+    params = scene.params.copy()
+    params["particles_count"] = scene.n
     steps = 0
     t0 = time.perf_counter()
     while steps < max_steps:
-        parts, _, _, _ = oracle.run(parts, scene.params, scene.terms, scene.normals,
+        parts, _, _, _ = oracle.run(parts, params, scene.terms, scene.normals,
                                     scene.vertices, scene.indices, 1)
         steps += 1
         if time.perf_counter() - t0 > budget_s / 2:
@@ -62,17 +69,17 @@ def cpu_baseline(scene, budget_s=20.0, max_steps=3):
         "unit": "particle-steps/s",
         "cores": oracle.threads(),
         "kind": "port",
-        "sample": f"{steps} substep(s) of the {scene.n}-particle dam break from its "
-                  f"initial state, {dt:.1f} s on the host",
+        "sample": f"{steps} substep(s) of {label} from its initial state, {dt:.1f} s on the host",
     }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=900)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--particles", type=int, default=1048576)
+    ap.add_argument("--workload", choices=["dam_break", "heightfield"], default="dam_break")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -85,8 +92,20 @@ def main():
 
     from libclsph_amd import native, scenes
 
-    n = args.particles
-    scene = scenes.dam_break(n)
+    defaults = {"dam_break": (1048576, 900, 100), "heightfield": (4194304, 450, 50)}
+    n0, k0, w0 = defaults[args.workload]
+    n = args.particles or n0
+    args.steps = k0 if args.steps is None else args.steps
+    args.warmup = w0 if args.warmup is None else args.warmup
+    if args.workload == "dam_break":
+        scene = scenes.dam_break(n)
+        metric = "particle-steps/s @1M particles (plain SPH substep)"
+        cfg = {"workload": "dam_break", "particles": n}
+    else:
+        scene = scenes.heightfield_pour(n)
+        metric = "particle-steps/s @4M particles poured on a 32,768-triangle heightfield"
+        cfg = {"workload": "heightfield_pour", "particles": n,
+               "triangles": int(len(scene.indices) // 3)}
     ctx = native.Context(0)
     st = native.State(ctx, n)
     st.set_params(scene.params, scene.terms)
@@ -122,8 +141,15 @@ def main():
     finite = bool(np.isfinite(final["position"][:, :3]).all())
     capped = st.bounce_capped()
 
+    cfg.update({
+        "dt": float(scene.params["time_delta"]),
+        "timed_substeps": f"{args.warmup + 1}-{args.warmup + args.steps}",
+        "grid_cell_count_end": cells,
+        "state_finite": finite,
+        "bounce_capped_last": capped,
+    })
     out = {
-        "metric": "particle-steps/s @1M particles (plain SPH substep)",
+        "metric": metric,
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -135,15 +161,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": "dam_break",
-            "particles": n,
-            "dt": float(scene.params["time_delta"]),
-            "timed_substeps": f"{args.warmup + 1}-{args.warmup + args.steps}",
-            "grid_cell_count_end": cells,
-            "state_finite": finite,
-            "bounce_capped_last": capped,
-        },
+        "config": cfg,
         "stages_ms": {k: v["ms"] for k, v in stages.items()},
         "roofline": {
             "kernel": dom,
@@ -156,7 +174,14 @@ def main():
         },
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(scene)
+        if args.workload == "dam_break":
+            out["cpu_baseline"] = cpu_baseline(scene, f"the {n}-particle dam break")
+        else:
+            sub = scenes.Scene(scene.particles[:65536].copy(), scene.params.copy(), scene.terms,
+                               scene.normals, scene.vertices, scene.indices)
+            out["cpu_baseline"] = cpu_baseline(
+                sub, f"the bottom 65,536 particles of the {n}-particle column (every particle "
+                     f"tested against all {len(scene.indices) // 3} triangles)")
     print(json.dumps(out), flush=True)
 
 

@@ -79,6 +79,16 @@ int clsph_scene_upload(clsph_state* st, const float* face_normals,
                        const float* vertices, uint32_t vertex_count,
                        const uint32_t* indices, uint32_t face_count);
 
+/* Collision acceleration structure for the uploaded mesh. `cell_size` > 0
+ * builds a uniform grid of that cell size over the mesh (each cell lists the
+ * triangles whose bounding box, padded by the rounding margin, meets it);
+ * 0 picks a size from the mean triangle extent; < 0 removes the grid so
+ * every substep tests every triangle, as advection_collision does
+ * (collisions.cl:21-87). clsph_scene_upload builds a grid by itself for
+ * meshes of more than 64 triangles. Hits are the same either way: the
+ * nearest hit, ties going to the highest triangle index. */
+int clsph_scene_grid(clsph_state* st, float cell_size);
+
 /* ---- stages of one substep (simulate_single_frame, sph_simulation.cpp:173-344) */
 /* AABB + padding + grid dimensions: sph_simulation.cpp:199-252 */
 int clsph_grid(clsph_state* st);
@@ -145,6 +155,25 @@ uint32_t clsph_make_block(clsph_particle* out, uint32_t n, float spacing,
  * expressions (sph_simulation.cpp:499-505: float pow(h, 9.f) / pow(h, 6.f),
  * double M_PI arithmetic, stored as float). */
 void clsph_kernel_terms(float h, clsph_precomputed_kernel_values* out);
+/* Heightfield over [x0, x0+size] x [z0, z0+size]: res x res quads, 2
+ * triangles each (2 res^2 faces, (res+1)^2 vertices), heights
+ * y = amplitude * N(x, z) with N in [0, 1) a `octaves`-octave value noise
+ * (lattice values from a counter-based generator seeded by `seed`, smooth
+ * bilinear interpolation, frequency doubling and amplitude halving per
+ * octave, base lattice of 4 cells across). Normals as scene.cpp:36-64
+ * computes them, oriented upward. Writes (res+1)^2*3 vertex floats, 6 res^2
+ * indices and 6 res^2 normal floats. */
+void clsph_make_heightfield(uint32_t res, float x0, float z0, float size, float amplitude,
+                            uint32_t octaves, uint64_t seed, float* vertices,
+                            uint32_t* indices, float* face_normals);
+/* Column of n particles on a cubic lattice at `spacing`: round(extent/spacing)
+ * sites along x and z from origin, stacked upward in y, jittered by
+ * U(+-jitter)*spacing, each with velocity (U(-vh, vh), -U(v0, v1), U(-vh, vh))
+ * from a counter-based generator seeded by `seed` (intermediate_velocity
+ * equal to velocity). Returns n. */
+uint32_t clsph_make_column(clsph_particle* out, uint32_t n, float spacing, const float origin[3],
+                           const float extent[3], float jitter, float vh, float v0, float v1,
+                           uint64_t seed);
 /* Axis-aligned box [lo, hi] as 12 triangles with inward unit normals:
  * fills 8*3 vertex floats, 12*3 indices and 12*3 normal floats. */
 void clsph_make_box(const float lo[3], const float hi[3], float* vertices,

@@ -14,6 +14,10 @@
 // cell id), which is the order the next substep starts from.
 #include "state.h"
 
+// No multiply-add fusion anywhere in this file, helpers included: the
+// reference evaluates these expressions as separate float operations.
+#pragma clang fp contract(off)
+
 namespace clsph {
 
 constexpr int kBlock = 256;
@@ -35,45 +39,107 @@ struct Mesh {
   const float* __restrict__ verts;
   const uint32_t* __restrict__ idx;
   uint32_t faces;
+  TriGrid grid;
+  const uint32_t* __restrict__ cell_start;
+  const uint32_t* __restrict__ cell_tris;
 };
 
-// detect_collision (collisions.cl:15-89): nearest hit along p0 -> p1; on equal
-// distance the later triangle wins.
-__device__ inline bool detect(V3 p0, V3 p1, const Mesh& m, V3& cpoint, V3& cnormal,
-                              float& depth) {
-#pragma clang fp contract(off)
-  bool hit = false;
+struct Hit {
+  bool hit;
+  uint32_t face;
+  V3 point, normal;
+  float depth;
+};
+
+// Segment p0 -> p1 against triangle f (collisions.cl:21-76): on a hit,
+// returns true with the intersection, the normal oriented along the motion
+// and the penetration depth |p1 - intersection|.
+__device__ inline bool face_hit(uint32_t f, V3 p0, V3 p1, V3 seg, float seg_len, const Mesh& m,
+                                V3& isect, V3& nrm, float& depth) {
+  nrm = v3(m.normals[3 * f + 0], m.normals[3 * f + 1], m.normals[3 * f + 2]);
+  if (dot(nrm, seg) / (len(nrm) * seg_len) <= 0.f) nrm = v3(-nrm.x, -nrm.y, -nrm.z);
+  const float* a = m.verts + 3 * m.idx[3 * f + 0];
+  const float* b = m.verts + 3 * m.idx[3 * f + 1];
+  const float* c = m.verts + 3 * m.idx[3 * f + 2];
+  V3 v0 = v3(a[0], a[1], a[2]);
+  V3 u = v3(b[0], b[1], b[2]) - v0;
+  V3 v = v3(c[0], c[1], c[2]) - v0;
+  float denom = dot(nrm, seg);
+  if (denom == 0.f) return false;
+  float r = dot(nrm, v0 - p0) / denom;
+  if (!(0.f <= r && r <= 1.f)) return false;
+  isect = p0 + r * seg;
+  V3 w = isect - v0;
+  float uv = dot(u, v), wv = dot(w, v), vv = dot(v, v), wu = dot(w, u), uu = dot(u, u);
+  float d2 = uv * uv - uu * vv;
+  float s = (uv * wv - vv * wu) / d2;
+  float t = (uv * wu - uu * wv) / d2;
+  if (!(s >= 0.f && t >= 0.f && s + t <= 1.f)) return false;
+  depth = len(p1 - isect);
+  return true;
+}
+
+// detect_collision (collisions.cl:15-89): the nearest hit along p0 -> p1; a
+// hit at equal distance replaces the earlier one, so the highest index among
+// the nearest wins. Without a grid every triangle is tested in index order
+// with the reference's test; with a grid only the triangles listed in the
+// cells the segment's box meets are tested (the others cannot be hit), in
+// cell order, and "nearest, then highest index" is applied explicitly.
+__device__ inline Hit detect(V3 p0, V3 p1, const Mesh& m) {
+  Hit h;
+  h.hit = false;
+  h.face = 0;
+  h.depth = 0.f;
+  h.point = h.normal = v3(0.f, 0.f, 0.f);
   V3 seg = p1 - p0;
   float seg_len = len(seg);
-  for (uint32_t f = 0; f < m.faces; ++f) {
-    V3 nrm = v3(m.normals[3 * f + 0], m.normals[3 * f + 1], m.normals[3 * f + 2]);
-    if (dot(nrm, seg) / (len(nrm) * seg_len) <= 0.f) nrm = v3(-nrm.x, -nrm.y, -nrm.z);
-    const float* a = m.verts + 3 * m.idx[3 * f + 0];
-    const float* b = m.verts + 3 * m.idx[3 * f + 1];
-    const float* c = m.verts + 3 * m.idx[3 * f + 2];
-    V3 v0 = v3(a[0], a[1], a[2]);
-    V3 u = v3(b[0], b[1], b[2]) - v0;
-    V3 v = v3(c[0], c[1], c[2]) - v0;
-    float denom = dot(nrm, seg);
-    if (denom == 0.f) continue;
-    float r = dot(nrm, v0 - p0) / denom;
-    if (0.f <= r && r <= 1.f) {
-      V3 isect = p0 + r * seg;
-      V3 w = isect - v0;
-      float uv = dot(u, v), wv = dot(w, v), vv = dot(v, v), wu = dot(w, u), uu = dot(u, u);
-      float d2 = uv * uv - uu * vv;
-      float s = (uv * wv - vv * wu) / d2;
-      float t = (uv * wu - uu * wv) / d2;
-      if (s >= 0.f && t >= 0.f && s + t <= 1.f) {
-        if (hit && len(p0 - isect) > len(p0 - cpoint)) continue;
-        cnormal = nrm;
-        cpoint = isect;
-        depth = len(p1 - isect);
-        hit = true;
-      }
+  if (!m.grid.enabled) {
+    for (uint32_t f = 0; f < m.faces; ++f) {
+      V3 isect, nrm;
+      float depth;
+      if (!face_hit(f, p0, p1, seg, seg_len, m, isect, nrm, depth)) continue;
+      if (h.hit && len(p0 - isect) > len(p0 - h.point)) continue;
+      h.hit = true;
+      h.face = f;
+      h.point = isect;
+      h.normal = nrm;
+      h.depth = depth;
     }
+    return h;
   }
-  return hit;
+  float lo[3] = {fminf(p0.x, p1.x), fminf(p0.y, p1.y), fminf(p0.z, p1.z)};
+  float hi[3] = {fmaxf(p0.x, p1.x), fmaxf(p0.y, p1.y), fmaxf(p0.z, p1.z)};
+  uint32_t clo[3], chi[3];
+  for (int a = 0; a < 3; ++a) {
+    float cl = floorf((lo[a] - m.grid.origin[a]) * m.grid.inv_cell);
+    float ch = floorf((hi[a] - m.grid.origin[a]) * m.grid.inv_cell);
+    float top = (float)(m.grid.dims[a] - 1);
+    // the grid covers every triangle with margin: a box outside it hits nothing
+    if (!(ch >= 0.f) || !(cl <= top)) return h;
+    clo[a] = (uint32_t)fmaxf(cl, 0.f);
+    chi[a] = (uint32_t)fminf(ch, top);
+  }
+  for (uint32_t z = clo[2]; z <= chi[2]; ++z)
+    for (uint32_t y = clo[1]; y <= chi[1]; ++y)
+      for (uint32_t x = clo[0]; x <= chi[0]; ++x) {
+        uint32_t c = (z * m.grid.dims[1] + y) * m.grid.dims[0] + x;
+        for (uint32_t q = m.cell_start[c]; q < m.cell_start[c + 1]; ++q) {
+          uint32_t f = m.cell_tris[q];
+          V3 isect, nrm;
+          float depth;
+          if (!face_hit(f, p0, p1, seg, seg_len, m, isect, nrm, depth)) continue;
+          if (h.hit) {
+            float d = len(p0 - isect), db = len(p0 - h.point);
+            if (d > db || (d == db && f <= h.face)) continue;
+          }
+          h.hit = true;
+          h.face = f;
+          h.point = isect;
+          h.normal = nrm;
+          h.depth = depth;
+        }
+      }
+  return h;
 }
 
 __device__ inline float wave_min(float v) {
@@ -90,7 +156,6 @@ __global__ __launch_bounds__(kBlock) void k_advect(
     const float4* __restrict__ acc_s, const uint32_t* __restrict__ slot, Mesh mesh,
     DevParams* __restrict__ dp, float4* __restrict__ pos, float4* __restrict__ vel,
     float4* __restrict__ vh, float* __restrict__ aabb_partial, uint32_t n) {
-#pragma clang fp contract(off)
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   bool valid = i < n;
   float4 outp = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -114,14 +179,13 @@ __global__ __launch_bounds__(kBlock) void k_advect(
       rpos = np;
       rvel = nv;
       float elapsed = time_to_go;
-      V3 cp = v3(0.f, 0.f, 0.f), cn = cp;
-      float depth = 0.f;
-      happened = detect(p, np, mesh, cp, cn, depth);
+      Hit hh = detect(p, np, mesh);
+      happened = hh.hit;
       if (happened) {
         // respond (collisions.cl:91-104)
-        rpos = cp - cn * 0.001f;
-        float k = (1.f + e * depth / (time_to_go * len(rvel))) * dot(rvel, cn);
-        rvel = rvel - k * cn;
+        rpos = hh.point - hh.normal * 0.001f;
+        float k = (1.f + e * hh.depth / (time_to_go * len(rvel))) * dot(rvel, hh.normal);
+        rvel = rvel - k * hh.normal;
         elapsed = time_to_go * (len(rpos - p) / len(np - p));
       }
       p = rpos;
@@ -176,7 +240,8 @@ __global__ __launch_bounds__(kBlock) void k_advect(
 
 void launch_advect(clsph_state* s) {
   if (s->n == 0) return;
-  Mesh m{s->face_normals, s->vertices, s->indices, s->face_count};
+  Mesh m{s->face_normals, s->vertices, s->indices, s->face_count, s->tgrid, s->tgrid_start,
+         s->tgrid_tris};
   uint32_t blocks = (s->n + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_advect, dim3(blocks), dim3(kBlock), 0, s->ctx->stream, s->pos_s,
                      s->vh_s, s->acc_s, s->slot, m, s->dp, s->pos, s->vel, s->vh,

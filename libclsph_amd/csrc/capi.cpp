@@ -2,6 +2,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -65,7 +67,8 @@ int device_errors(clsph_state* st) {
 }
 
 void free_state_buffers(clsph_state* st) {
-  void* bufs[] = {st->pos,  st->vel,   st->vh,    st->pos_s, st->vel_s, st->vh_s,
+  void* bufs[] = {st->tgrid_start, st->tgrid_tris,
+                  st->pos,  st->vel,   st->vh,    st->pos_s, st->vel_s, st->vh_s,
                   st->acc_s, st->rho_s, st->prs_s, st->fa, st->fb, st->key_a, st->val_a,
                   st->key_b, st->val_b, st->hist,  st->digit_base, st->table,
                   st->aabb_partial, st->face_normals, st->vertices, st->indices, st->dp,
@@ -121,6 +124,107 @@ int run_stage(clsph_state* st, int stage) {
   }
   st->stage = stage;
   return check_launch(st->ctx);
+}
+
+// Uniform grid over the mesh for collision queries: cell c lists every
+// triangle whose bounding box, padded by a rounding margin, meets it. Built
+// on the host once per mesh (the mesh is static).
+int build_tri_grid(clsph_state* st, const float* verts, const uint32_t* idx, uint32_t faces,
+                   float cell_size) {
+  clsph_ctx* ctx = st->ctx;
+  if (st->tgrid_start) (void)hipFree(st->tgrid_start);
+  if (st->tgrid_tris) (void)hipFree(st->tgrid_tris);
+  st->tgrid_start = nullptr;
+  st->tgrid_tris = nullptr;
+  st->tgrid.enabled = 0;
+  if (faces == 0 || cell_size < 0.f) return CLSPH_OK;
+  float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+  double ext_sum = 0.0;
+  for (uint32_t f = 0; f < faces; ++f) {
+    float tlo[3] = {1e30f, 1e30f, 1e30f}, thi[3] = {-1e30f, -1e30f, -1e30f};
+    for (int k = 0; k < 3; ++k)
+      for (int a = 0; a < 3; ++a) {
+        float v = verts[3 * idx[3 * f + k] + a];
+        tlo[a] = std::min(tlo[a], v);
+        thi[a] = std::max(thi[a], v);
+      }
+    float e = 0.f;
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], tlo[a]);
+      hi[a] = std::max(hi[a], thi[a]);
+      e = std::max(e, thi[a] - tlo[a]);
+    }
+    ext_sum += e;
+  }
+  if (!(cell_size > 0.f)) cell_size = (float)(2.0 * ext_sum / faces);
+  float span = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+  if (!(cell_size > 0.f)) cell_size = span > 0.f ? span : 1.f;
+  // Rounding margin: hit points are computed in FP32 and may stray from the
+  // exact triangle by a few ulps of the coordinates involved.
+  float mag = 0.f;
+  for (int a = 0; a < 3; ++a) mag = std::max(mag, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
+  float pad = 1e-5f * (mag + cell_size) + 1e-6f;
+  uint32_t dims[3];
+  for (;;) {
+    uint64_t total = 1;
+    for (int a = 0; a < 3; ++a) {
+      float ext = (hi[a] - lo[a]) + 2.f * (pad + cell_size);
+      dims[a] = (uint32_t)std::max(1.0, std::ceil((double)ext / (double)cell_size));
+      total *= dims[a];
+    }
+    if (total <= (1ull << 25)) break;
+    cell_size *= 1.5f;
+  }
+  float origin[3];
+  for (int a = 0; a < 3; ++a) origin[a] = lo[a] - pad - cell_size;
+  float inv = 1.f / cell_size;
+  auto cell_of = [&](float v, int a) {
+    float c = std::floor((v - origin[a]) * inv);
+    c = std::max(0.f, std::min(c, (float)(dims[a] - 1)));
+    return (uint32_t)c;
+  };
+  uint64_t ncells = (uint64_t)dims[0] * dims[1] * dims[2];
+  std::vector<uint32_t> count(ncells + 1, 0);
+  std::vector<uint32_t> range(6ull * faces);
+  for (uint32_t f = 0; f < faces; ++f) {
+    float tlo[3] = {1e30f, 1e30f, 1e30f}, thi[3] = {-1e30f, -1e30f, -1e30f};
+    for (int k = 0; k < 3; ++k)
+      for (int a = 0; a < 3; ++a) {
+        float v = verts[3 * idx[3 * f + k] + a];
+        tlo[a] = std::min(tlo[a], v);
+        thi[a] = std::max(thi[a], v);
+      }
+    for (int a = 0; a < 3; ++a) {
+      range[6 * f + a] = cell_of(tlo[a] - pad, a);
+      range[6 * f + 3 + a] = cell_of(thi[a] + pad, a);
+    }
+    for (uint32_t z = range[6 * f + 2]; z <= range[6 * f + 5]; ++z)
+      for (uint32_t y = range[6 * f + 1]; y <= range[6 * f + 4]; ++y)
+        for (uint32_t x = range[6 * f + 0]; x <= range[6 * f + 3]; ++x)
+          count[((uint64_t)z * dims[1] + y) * dims[0] + x]++;
+  }
+  std::vector<uint32_t> start(ncells + 1, 0);
+  for (uint64_t c = 0; c < ncells; ++c) start[c + 1] = start[c] + count[c];
+  std::vector<uint32_t> tris(std::max<uint32_t>(start[ncells], 1));
+  std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+  for (uint32_t f = 0; f < faces; ++f)
+    for (uint32_t z = range[6 * f + 2]; z <= range[6 * f + 5]; ++z)
+      for (uint32_t y = range[6 * f + 1]; y <= range[6 * f + 4]; ++y)
+        for (uint32_t x = range[6 * f + 0]; x <= range[6 * f + 3]; ++x)
+          tris[fill[((uint64_t)z * dims[1] + y) * dims[0] + x]++] = f;
+  int rc = dalloc(ctx, &st->tgrid_start, ncells + 1);
+  if (!rc) rc = dalloc(ctx, &st->tgrid_tris, tris.size());
+  if (rc) return rc;
+  HIPCK(ctx, hipMemcpy(st->tgrid_start, start.data(), 4 * (ncells + 1), hipMemcpyHostToDevice));
+  HIPCK(ctx, hipMemcpy(st->tgrid_tris, tris.data(), 4 * tris.size(), hipMemcpyHostToDevice));
+  for (int a = 0; a < 3; ++a) {
+    st->tgrid.origin[a] = origin[a];
+    st->tgrid.dims[a] = dims[a];
+  }
+  st->tgrid.inv_cell = inv;
+  st->tgrid.enabled = 1;
+  st->graph_dirty = true;
+  return CLSPH_OK;
 }
 
 uint64_t splitmix64(uint64_t x) {
@@ -387,7 +491,20 @@ int clsph_scene_upload(clsph_state* st, const float* face_normals, const float* 
   st->face_count = face_count;
   st->vert_floats = 3 * vertex_count;
   st->graph_dirty = true;
-  return CLSPH_OK;
+  return build_tri_grid(st, vertices, indices, face_count, face_count > 64 ? 0.f : -1.f);
+}
+
+int clsph_scene_grid(clsph_state* st, float cell_size) {
+  if (!st) return CLSPH_EINVAL;
+  HIPCK(st->ctx, hipStreamSynchronize(st->ctx->stream));
+  std::vector<float> verts(st->vert_floats);
+  std::vector<uint32_t> idx(3ull * st->face_count);
+  if (st->vert_floats)
+    HIPCK(st->ctx, hipMemcpy(verts.data(), st->vertices, 4ull * st->vert_floats, hipMemcpyDeviceToHost));
+  if (st->face_count)
+    HIPCK(st->ctx, hipMemcpy(idx.data(), st->indices, 12ull * st->face_count, hipMemcpyDeviceToHost));
+  st->graph_dirty = true;
+  return build_tri_grid(st, verts.data(), idx.data(), st->face_count, cell_size);
 }
 
 int clsph_grid(clsph_state* st) { return st ? run_stage(st, 0) : CLSPH_EINVAL; }
@@ -593,6 +710,88 @@ uint32_t clsph_make_block(clsph_particle* out, uint32_t n, float spacing, const 
     }
   }
   return count;
+}
+
+// Value noise: lattice values in [0, 1) from the counter-based generator,
+// blended with the smoothstep of the fractional position.
+static double lattice_value(uint64_t seed, uint32_t octave, int64_t ix, int64_t iz) {
+  uint64_t key = seed ^ (0x9E3779B97F4A7C15ull * (octave + 1));
+  key = splitmix64(key ^ ((uint64_t)(ix & 0xFFFFFFFF) << 32 | (uint64_t)(iz & 0xFFFFFFFF)));
+  return (double)(key >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static double value_noise(double x, double z, uint32_t octaves, uint64_t seed) {
+  double sum = 0.0, amp = 1.0, norm = 0.0, freq = 4.0;
+  for (uint32_t o = 0; o < octaves; ++o) {
+    double fx = x * freq, fz = z * freq;
+    double x0 = std::floor(fx), z0 = std::floor(fz);
+    double tx = fx - x0, tz = fz - z0;
+    tx = tx * tx * (3.0 - 2.0 * tx);
+    tz = tz * tz * (3.0 - 2.0 * tz);
+    int64_t ix = (int64_t)x0, iz = (int64_t)z0;
+    double a = lattice_value(seed, o, ix, iz), b = lattice_value(seed, o, ix + 1, iz);
+    double c = lattice_value(seed, o, ix, iz + 1), d = lattice_value(seed, o, ix + 1, iz + 1);
+    double v = (a * (1 - tx) + b * tx) * (1 - tz) + (c * (1 - tx) + d * tx) * tz;
+    sum += amp * v;
+    norm += amp;
+    amp *= 0.5;
+    freq *= 2.0;
+  }
+  return norm > 0 ? sum / norm : 0.0;
+}
+
+void clsph_make_heightfield(uint32_t res, float x0, float z0, float size, float amplitude,
+                            uint32_t octaves, uint64_t seed, float* vertices, uint32_t* indices,
+                            float* face_normals) {
+  uint32_t nv = res + 1;
+  for (uint32_t j = 0; j < nv; ++j)
+    for (uint32_t i = 0; i < nv; ++i) {
+      double u = (double)i / res, w = (double)j / res;
+      float* v = vertices + 3 * (j * nv + i);
+      v[0] = (float)(x0 + u * size);
+      v[2] = (float)(z0 + w * size);
+      v[1] = (float)(amplitude * value_noise(u, w, octaves, seed));
+    }
+  uint32_t f = 0;
+  for (uint32_t j = 0; j < res; ++j)
+    for (uint32_t i = 0; i < res; ++i) {
+      uint32_t a = j * nv + i, b = a + 1, c = a + nv, d = c + 1;
+      // (a, c, b) and (b, c, d): (B - A) x (C - A) points up (+y)
+      uint32_t tri[2][3] = {{a, c, b}, {b, c, d}};
+      for (int t = 0; t < 2; ++t, ++f) {
+        const float* A = vertices + 3 * tri[t][0];
+        const float* B = vertices + 3 * tri[t][1];
+        const float* C = vertices + 3 * tri[t][2];
+        float ux = B[0] - A[0], uy = B[1] - A[1], uz = B[2] - A[2];
+        float vx = C[0] - A[0], vy = C[1] - A[1], vz = C[2] - A[2];
+        float nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+        float l = sqrtf(nx * nx + ny * ny + nz * nz);
+        face_normals[3 * f + 0] = nx / l;
+        face_normals[3 * f + 1] = ny / l;
+        face_normals[3 * f + 2] = nz / l;
+        for (int k = 0; k < 3; ++k) indices[3 * f + k] = tri[t][k];
+      }
+    }
+}
+
+uint32_t clsph_make_column(clsph_particle* out, uint32_t n, float spacing, const float origin[3],
+                           const float extent[3], float jitter, float vh, float v0, float v1,
+                           uint64_t seed) {
+  uint32_t got = clsph_make_block(out, n, spacing, origin, extent, jitter, seed);
+  for (uint32_t k = 0; k < got; ++k) {
+    double u[3];
+    for (int a = 0; a < 3; ++a) {
+      uint64_t r = splitmix64((seed * 0x2545F4914F6CDD1Dull) ^ (3ull * k + (uint64_t)a + 1));
+      u[a] = (double)(r >> 40) * (1.0 / 16777216.0);
+    }
+    float vel[3] = {(float)((2 * u[0] - 1) * vh), (float)(-(v0 + u[1] * (v1 - v0))),
+                    (float)((2 * u[2] - 1) * vh)};
+    for (int a = 0; a < 3; ++a) {
+      out[k].velocity.s[a] = vel[a];
+      out[k].intermediate_velocity.s[a] = vel[a];
+    }
+  }
+  return got;
 }
 
 void clsph_kernel_terms(float h, clsph_precomputed_kernel_values* out) {

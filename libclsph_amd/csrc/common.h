@@ -42,6 +42,14 @@ struct DevParams {
   uint32_t max_bounces;
 };
 
+// Uniform grid over the collision mesh (cells x fastest, then y, then z).
+struct TriGrid {
+  float origin[3];
+  float inv_cell;
+  uint32_t dims[3];
+  uint32_t enabled;
+};
+
 __host__ __device__ inline uint32_t spread10(uint32_t v) {
   v = (v | (v << 16)) & 0x030000FFu;
   v = (v | (v << 8)) & 0x0300F00Fu;

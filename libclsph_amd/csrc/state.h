@@ -63,6 +63,11 @@ struct clsph_state {
   uint32_t* indices = nullptr;
   uint32_t face_count = 0;
   uint32_t vert_floats = 0;
+  // Uniform triangle grid (CSR: tgrid_start[cell]..tgrid_start[cell+1] index
+  // tgrid_tris), see clsph_scene_grid.
+  clsph::TriGrid tgrid{};
+  uint32_t* tgrid_start = nullptr;
+  uint32_t* tgrid_tris = nullptr;
 
   clsph::DevParams* dp = nullptr;  // device parameter block
   clsph::DevParams host_dp{};      // host copy of the host-set fields

@@ -70,6 +70,12 @@ def _load():
                                     ctypes.c_uint64]),
         "clsph_kernel_terms": (None, [ctypes.c_float, _vp]),
         "clsph_make_box": (None, [_vp, _vp, _vp, _vp, _vp]),
+        "clsph_scene_grid": (_i32, [_vp, ctypes.c_float]),
+        "clsph_make_heightfield": (None, [_u32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                          ctypes.c_float, _u32, ctypes.c_uint64, _vp, _vp, _vp]),
+        "clsph_make_column": (_u32, [_vp, _u32, ctypes.c_float, _vp, _vp, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_uint64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -110,6 +116,26 @@ def make_box(lo, hi):
     nrm = np.zeros(36, dtype=np.float32)
     lib.clsph_make_box(_ptr(lo), _ptr(hi), _ptr(verts), _ptr(idx), _ptr(nrm))
     return nrm, verts, idx
+
+
+def make_heightfield(res, x0, z0, size, amplitude, octaves, seed):
+    verts = np.zeros(3 * (res + 1) ** 2, dtype=np.float32)
+    idx = np.zeros(6 * res * res, dtype=np.uint32)
+    nrm = np.zeros(6 * res * res, dtype=np.float32)
+    lib.clsph_make_heightfield(res, ctypes.c_float(x0), ctypes.c_float(z0), ctypes.c_float(size),
+                               ctypes.c_float(amplitude), octaves, ctypes.c_uint64(seed),
+                               _ptr(verts), _ptr(idx), _ptr(nrm))
+    return nrm, verts, idx
+
+
+def make_column(n, spacing, origin, extent, jitter, vh, v0, v1, seed):
+    out = layout.particles(n)
+    o = np.asarray(origin, dtype=np.float32)
+    e = np.asarray(extent, dtype=np.float32)
+    got = lib.clsph_make_column(_ptr(out), n, ctypes.c_float(spacing), _ptr(o), _ptr(e),
+                                ctypes.c_float(jitter), ctypes.c_float(vh), ctypes.c_float(v0),
+                                ctypes.c_float(v1), ctypes.c_uint64(seed))
+    return out[:got]
 
 
 # ---------------- device context / state ----------------
@@ -195,6 +221,10 @@ class State:
         self.ctx.check(lib.clsph_scene_upload(self.h, _ptr(normals), _ptr(vertices),
                                               len(vertices) // 3, _ptr(indices),
                                               len(indices) // 3))
+
+    def scene_grid(self, cell_size):
+        """Triangle grid for collisions: > 0 cell size, 0 automatic, < 0 none."""
+        self.ctx.check(lib.clsph_scene_grid(self.h, ctypes.c_float(cell_size)))
 
     def stage(self, name):
         fn = {"grid": lib.clsph_grid, "hash": lib.clsph_hash, "sort": lib.clsph_sort,

@@ -1,4 +1,5 @@
-"""Synthetic scenes: the dam-break configuration in its several sizes.
+"""Synthetic scenes: the dam-break configuration in its several sizes, and
+the column poured onto a heightfield.
 
 Dam break: a fluid block 0.4 x 0.8 x 0.4 m on a cubic lattice at spacing
 s = cbrt(0.128 / N) (so N particles fill it; h = 2 s, i.e. spacing 0.5 h),
@@ -69,4 +70,50 @@ def dam_break(n, seed=1234, jitter=0.01, block=(0.4, 0.8, 0.4), box=1.0, **kw):
     params = fluid_params(n, h, mass, **kw)
     terms = native.kernel_terms(float(h))
     normals, verts, idx = native.make_box((0.0, 0.0, 0.0), (box, box, box))
+    return Scene(parts, params, terms, normals, verts, idx)
+
+
+def merge_meshes(*meshes):
+    """Concatenates (normals, vertices, indices) triples into one mesh."""
+    normals, verts, idx = [], [], []
+    base = 0
+    for nrm, v, i in meshes:
+        normals.append(np.asarray(nrm, np.float32))
+        verts.append(np.asarray(v, np.float32))
+        idx.append(np.asarray(i, np.uint32) + base)
+        base += len(v) // 3
+    return np.concatenate(normals), np.concatenate(verts), np.concatenate(idx)
+
+
+def terrain(res=128, amplitude=0.1, octaves=4, seed=77, walls=True):
+    """A res x res heightfield over [0, 1]^2 (2 res^2 triangles, heights from
+    4-octave value noise with amplitude 0.1 m) inside the side walls and
+    floor of the box [0, 1] x [-0.1, 2] x [0, 1] (the top is open)."""
+    hf = native.make_heightfield(res, 0.0, 0.0, 1.0, amplitude, octaves, seed)
+    if not walls:
+        return hf
+    nrm, v, idx = native.make_box((0.0, -0.1, 0.0), (1.0, 2.0, 1.0))
+    tri = idx.reshape(-1, 3)
+    V = v.reshape(-1, 3)
+    top = np.all(V[tri][:, :, 1] > 1.99, axis=1)  # drop the two top triangles
+    keep = ~top
+    box = (nrm.reshape(-1, 3)[keep].ravel(), v, tri[keep].ravel())
+    return merge_meshes(hf, box)
+
+
+def heightfield_pour(n, seed=4321, column=(0.3, 0.3), base=0.15, dt=5e-5, **kw):
+    """n particles in a column (0.3 x 0.3 m footprint centred over the
+    terrain, bottom at 0.15 m) with seeded random velocities
+    (U(+-0.25) m/s sideways, 0.5-1.5 m/s downward), at the dam-break lattice
+    spacing s = cbrt(0.128 / n), h = 2 s, above the terrain of terrain()."""
+    spacing = np.float32((0.128 / n) ** (1.0 / 3.0))
+    h = np.float32(2.0) * spacing
+    mass = np.float32(1000.0 * float(spacing) ** 3)
+    x0 = 0.5 - column[0] / 2
+    z0 = 0.5 - column[1] / 2
+    parts = native.make_column(n, float(spacing), (x0, base, z0), (column[0], 0.0, column[1]),
+                               0.01, 0.25, 0.5, 1.5, seed)
+    params = fluid_params(n, h, mass, dt=dt, **kw)
+    terms = native.kernel_terms(float(h))
+    normals, verts, idx = terrain()
     return Scene(parts, params, terms, normals, verts, idx)

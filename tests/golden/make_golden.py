@@ -6,6 +6,9 @@
                        density, pressure, acceleration, advected state)
   dam16k_series.npz  : the 16,384-particle dam break run 1,000 substeps:
                        mean density and kinetic energy after each substep
+  pour8k_series.npz  : 8,192 particles poured onto the 32,768-triangle
+                       heightfield (dt 2 ms), 50 substeps: mean density and
+                       kinetic energy after each substep
 
     python tests/golden/make_golden.py [--skip-series]
 """
@@ -55,10 +58,24 @@ def series(steps=1000):
                         kinetic_energy=ke)
 
 
+def pour_series(steps=50):
+    sc = scenes.heightfield_pour(8192, dt=2e-3)
+    t0 = time.time()
+    _, rho, ke, _ = oracle.run(sc.particles, sc.params, sc.terms, sc.normals, sc.vertices,
+                               sc.indices, steps)
+    print(f"pour series: {steps} substeps in {time.time() - t0:.0f} s")
+    np.savez_compressed(os.path.join(HERE, "pour8k_series.npz"), mean_density=rho,
+                        kinetic_energy=ke)
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-series", action="store_true")
+    ap.add_argument("--only", choices=["step0", "series", "pour"])
     a = ap.parse_args()
-    step0()
-    if not a.skip_series:
+    if a.only in (None, "step0"):
+        step0()
+    if a.only in (None, "series") and not a.skip_series:
         series()
+    if a.only in (None, "pour") and not a.skip_series:
+        pour_series()

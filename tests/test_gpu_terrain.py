@@ -1,0 +1,116 @@
+"""Collision against the 32,768-triangle heightfield: the triangle grid gives
+the brute-force answer (nearest hit, ties to the highest index), on the GPU
+and against the oracle's brute force over every triangle."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from libclsph_amd import layout, native, scenes
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = native.Context(0)
+    yield c
+    c.close()
+
+
+def terrain_height(verts, res, x, z):
+    """Height of the heightfield surface (piecewise-linear triangles)."""
+    V = verts[: 3 * (res + 1) ** 2].reshape(res + 1, res + 1, 3)
+    fx, fz = x * res, z * res
+    i = np.minimum(fx.astype(int), res - 1)
+    j = np.minimum(fz.astype(int), res - 1)
+    tx, tz = fx - i, fz - j
+    a, b = V[j, i, 1], V[j, i + 1, 1]
+    c, d = V[j + 1, i, 1], V[j + 1, i + 1, 1]
+    lower = tx + tz <= 1
+    return np.where(lower, a + (b - a) * tx + (c - a) * tz, d + (c - d) * (1 - tx) + (b - d) * (1 - tz))
+
+
+def sparse_drop(seed=3):
+    """4096 isolated particles (lattice 15 mm apart, h = 5 mm: no neighbours,
+    so each sees exactly a = g) 0.1-2 mm above the terrain, moving down at
+    2 m/s with a sideways drift; one 2 ms substep crosses the surface."""
+    rng = np.random.default_rng(seed)
+    normals, verts, idx = scenes.terrain()
+    g = (np.arange(64) + 0.5) / 64
+    X, Z = np.meshgrid(g, g, indexing="ij")
+    x = (X.ravel() + rng.uniform(-0.002, 0.002, 4096)).astype(np.float64)
+    z = (Z.ravel() + rng.uniform(-0.002, 0.002, 4096)).astype(np.float64)
+    y = terrain_height(verts, 128, x, z) + rng.uniform(1e-4, 2e-3, 4096)
+    parts = layout.particles(4096)
+    parts["position"][:, 0] = x
+    parts["position"][:, 1] = y
+    parts["position"][:, 2] = z
+    v = np.stack([rng.uniform(-0.5, 0.5, 4096), np.full(4096, -2.0),
+                  rng.uniform(-0.5, 0.5, 4096)], axis=1)
+    parts["velocity"][:, :3] = v
+    parts["intermediate_velocity"][:, :3] = v
+    p = scenes.fluid_params(4096, 0.005, 1e-4, dt=2e-3)
+    p["restitution"] = np.float32(0.3)
+    return scenes.Scene(parts, p, native.kernel_terms(0.005), normals, verts, idx)
+
+
+def run_one(ctx, sc, grid):
+    st = native.State(ctx, sc.n, table_cap=1 << 22)
+    st.set_params(sc.params, sc.terms)
+    st.scene(sc.normals, sc.vertices, sc.indices)
+    st.scene_grid(grid)
+    st.upload(sc.particles)
+    for s in ("grid", "hash", "sort", "cell_table", "density", "forces", "advect"):
+        st.stage(s)
+    out = st.download()
+    cap = st.bounce_capped()
+    st.close()
+    return out, cap
+
+
+def test_grid_equals_brute_force_and_oracle(ctx):
+    print("case terrain sparse drop", flush=True)
+    sc = sparse_drop()
+    grid_out, grid_cap = run_one(ctx, sc, 0.0)     # automatic grid
+    brute_out, brute_cap = run_one(ctx, sc, -1.0)  # every triangle
+    fine_out, _ = run_one(ctx, sc, 0.004)          # a finer grid
+    for f in ("position", "velocity", "intermediate_velocity"):
+        np.testing.assert_array_equal(grid_out[f], brute_out[f], err_msg=f)
+        np.testing.assert_array_equal(fine_out[f], brute_out[f], err_msg=f)
+    assert grid_cap == brute_cap
+    ref = oracle.stages(sc.particles, sc.params, sc.terms, sc.normals, sc.vertices, sc.indices)
+    # no neighbours: the oracle's acceleration is exactly gravity too
+    assert np.all(ref.forces["acceleration"][:, 1] == np.float32(-9.81))
+    moved = np.abs(ref.advected["position"][:, 1] - sc.particles["position"][:, 1]).max()
+    hits = np.sum(ref.advected["intermediate_velocity"][:, 1] > -2.0)
+    assert hits > 1000, "most particles should meet the terrain"
+    for f in ("position", "velocity", "intermediate_velocity"):
+        np.testing.assert_allclose(grid_out[f][:, :3], ref.advected[f][:, :3], rtol=0, atol=2e-7,
+                                   err_msg=f)
+    print(f"max drop {moved:.4f} m, {hits} terrain hits")
+
+
+def test_pour_series_against_oracle(ctx):
+    print("case pour8k 50 substeps", flush=True)
+    g = np.load(os.path.join(GOLDEN, "pour8k_series.npz"))
+    sc = scenes.heightfield_pour(8192, dt=2e-3)
+    st = native.State(ctx, sc.n, table_cap=1 << 22)
+    st.set_params(sc.params, sc.terms)
+    st.scene(sc.normals, sc.vertices, sc.indices)
+    st.upload(sc.particles)
+    M = float(sc.params["particle_mass"])
+    rho, ke = [], []
+    for _ in range(50):
+        st.step(1)
+        p = st.download()
+        rho.append(p["density"].astype(np.float64).mean())
+        v = p["velocity"][:, :3].astype(np.float64)
+        ke.append(0.5 * M * (v * v).sum())
+    rel_rho = np.abs(np.array(rho) / g["mean_density"] - 1)
+    rel_ke = np.abs(np.array(ke) / g["kinetic_energy"] - 1)
+    print(f"pour: max rel mean-density diff {rel_rho.max():.2e}, KE diff {rel_ke.max():.2e}")
+    assert rel_rho.max() <= 5e-3 and rel_ke.max() <= 5e-3
