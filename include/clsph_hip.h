@@ -50,8 +50,10 @@ int clsph_device_name(const clsph_ctx* ctx, char* buf, size_t len);
 int clsph_sync(clsph_ctx* ctx);
 
 /* ---- simulation state: replaces the front/back/sort_count cl::Buffers
- * (sph_simulation.cpp:373-380). `table_cap` is the cell-table capacity in
- * cells (0 = 2^30, enough for any 10-bit-per-axis grid). */
+ * (sph_simulation.cpp:373-380). A state holds up to n_cap particles of one
+ * scene. `table_cap` is the cell-table capacity in cells of side 2h
+ * (0 = max(2^20, 8 n_cap)); a substep whose grid has more cells fails with
+ * CLSPH_ETABLE. */
 int clsph_state_alloc(clsph_ctx* ctx, uint32_t n_cap, uint32_t table_cap,
                       clsph_state** out);
 int clsph_state_free(clsph_state* st);
@@ -108,6 +110,33 @@ int clsph_advect_collide(clsph_state* st);
 /* `nsteps` whole substeps (all stages above), replayed from one captured
  * HIP graph; replaces simulate_single_frame's body. */
 int clsph_step(clsph_state* st, uint32_t nsteps);
+
+/* ---- batches: many independent scenes advanced together. No counterpart
+ * in the reference, which runs one scene per process; every stage runs as one
+ * launch over all scenes. Scene s holds exactly counts[s] particles; each
+ * scene has its own parameters, mesh, grid and cell table, and its results
+ * are those of a state holding that scene alone. `table_cap` counts cells of
+ * all scenes together (0 = max(2^20, 8 * sum(counts))). The single-scene
+ * calls above act on scene 0. */
+int clsph_batch_alloc(clsph_ctx* ctx, uint32_t n_scenes, const uint32_t* counts,
+                      uint32_t table_cap, clsph_state** out);
+int clsph_batch_set_params(clsph_state* st, uint32_t scene,
+                           const clsph_simulation_parameters* params,
+                           const clsph_precomputed_kernel_values* terms, uint32_t max_bounces);
+int clsph_batch_upload(clsph_state* st, uint32_t scene, const clsph_particle* particles,
+                       uint32_t n);
+int clsph_batch_download(clsph_state* st, uint32_t scene, clsph_particle* particles,
+                         uint32_t n);
+int clsph_batch_scene_upload(clsph_state* st, uint32_t scene, const float* face_normals,
+                             const float* vertices, uint32_t vertex_count,
+                             const uint32_t* indices, uint32_t face_count);
+int clsph_batch_get_grid(clsph_state* st, uint32_t scene, clsph_simulation_parameters* params);
+int clsph_batch_bounce_capped(clsph_state* st, uint32_t scene, uint32_t* count);
+/* Work-group size of each stage's per-particle kernels (stage order as in
+ * clsph_time_stage; 64, 128, 256 or 512; 0 keeps the current one). The
+ * autotuner sets these; results do not depend on them. */
+int clsph_set_block_sizes(clsph_state* st, const uint32_t sizes[7]);
+int clsph_get_block_sizes(const clsph_state* st, uint32_t sizes[7]);
 
 /* ---- readback of stage results (for checks and the save delegates) ---- */
 /* Device-computed grid fields of the current substep, written into *params. */

@@ -12,6 +12,7 @@
 // the next substep's grid parameters need no separate pass over the state.
 // Results go to each particle's slot in the reference's order (stable by
 // cell id), which is the order the next substep starts from.
+#include "reduce.h"
 #include "state.h"
 
 // No multiply-add fusion anywhere in this file, helpers included: the
@@ -19,8 +20,6 @@
 #pragma clang fp contract(off)
 
 namespace clsph {
-
-constexpr int kBlock = 256;
 
 struct V3 {
   float x, y, z;
@@ -34,12 +33,12 @@ __device__ inline V3 operator/(V3 a, float s) { return v3(a.x / s, a.y / s, a.z 
 __device__ inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ inline float len(V3 a) { return sqrtf(dot(a, a)); }
 
+// The batch's concatenated meshes; a scene uses faces
+// [face_begin, face_begin + face_count) and its own triangle grid.
 struct Mesh {
   const float* __restrict__ normals;
   const float* __restrict__ verts;
   const uint32_t* __restrict__ idx;
-  uint32_t faces;
-  TriGrid grid;
   const uint32_t* __restrict__ cell_start;
   const uint32_t* __restrict__ cell_tris;
 };
@@ -85,7 +84,8 @@ __device__ inline bool face_hit(uint32_t f, V3 p0, V3 p1, V3 seg, float seg_len,
 // with the reference's test; with a grid only the triangles listed in the
 // cells the segment's box meets are tested (the others cannot be hit), in
 // cell order, and "nearest, then highest index" is applied explicitly.
-__device__ inline Hit detect(V3 p0, V3 p1, const Mesh& m) {
+__device__ inline Hit detect(V3 p0, V3 p1, const Mesh& m, const DevParams* __restrict__ dp) {
+  const TriGrid& grid = dp->tgrid;
   Hit h;
   h.hit = false;
   h.face = 0;
@@ -93,8 +93,8 @@ __device__ inline Hit detect(V3 p0, V3 p1, const Mesh& m) {
   h.point = h.normal = v3(0.f, 0.f, 0.f);
   V3 seg = p1 - p0;
   float seg_len = len(seg);
-  if (!m.grid.enabled) {
-    for (uint32_t f = 0; f < m.faces; ++f) {
+  if (!grid.enabled) {
+    for (uint32_t f = dp->face_begin; f < dp->face_begin + dp->face_count; ++f) {
       V3 isect, nrm;
       float depth;
       if (!face_hit(f, p0, p1, seg, seg_len, m, isect, nrm, depth)) continue;
@@ -111,9 +111,9 @@ __device__ inline Hit detect(V3 p0, V3 p1, const Mesh& m) {
   float hi[3] = {fmaxf(p0.x, p1.x), fmaxf(p0.y, p1.y), fmaxf(p0.z, p1.z)};
   uint32_t clo[3], chi[3];
   for (int a = 0; a < 3; ++a) {
-    float cl = floorf((lo[a] - m.grid.origin[a]) * m.grid.inv_cell);
-    float ch = floorf((hi[a] - m.grid.origin[a]) * m.grid.inv_cell);
-    float top = (float)(m.grid.dims[a] - 1);
+    float cl = floorf((lo[a] - grid.origin[a]) * grid.inv_cell);
+    float ch = floorf((hi[a] - grid.origin[a]) * grid.inv_cell);
+    float top = (float)(grid.dims[a] - 1);
     // the grid covers every triangle with margin: a box outside it hits nothing
     if (!(ch >= 0.f) || !(cl <= top)) return h;
     clo[a] = (uint32_t)fmaxf(cl, 0.f);
@@ -122,7 +122,7 @@ __device__ inline Hit detect(V3 p0, V3 p1, const Mesh& m) {
   for (uint32_t z = clo[2]; z <= chi[2]; ++z)
     for (uint32_t y = clo[1]; y <= chi[1]; ++y)
       for (uint32_t x = clo[0]; x <= chi[0]; ++x) {
-        uint32_t c = (z * m.grid.dims[1] + y) * m.grid.dims[0] + x;
+        uint32_t c = grid.cell_base + (z * grid.dims[1] + y) * grid.dims[0] + x;
         for (uint32_t q = m.cell_start[c]; q < m.cell_start[c + 1]; ++q) {
           uint32_t f = m.cell_tris[q];
           V3 isect, nrm;
@@ -142,22 +142,16 @@ __device__ inline Hit detect(V3 p0, V3 p1, const Mesh& m) {
   return h;
 }
 
-__device__ inline float wave_min(float v) {
-  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ inline float wave_max(float v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
-__global__ __launch_bounds__(kBlock) void k_advect(
+__global__ __launch_bounds__(kMaxBlock) void k_advect(
     const float4* __restrict__ pos_s, const float4* __restrict__ vh_s,
     const float4* __restrict__ acc_s, const uint32_t* __restrict__ slot, Mesh mesh,
-    DevParams* __restrict__ dp, float4* __restrict__ pos, float4* __restrict__ vel,
-    float4* __restrict__ vh, float* __restrict__ aabb_partial, uint32_t n) {
-  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  bool valid = i < n;
+    const BlockDesc* __restrict__ blocks, DevParams* __restrict__ dps,
+    float4* __restrict__ pos, float4* __restrict__ vel, float4* __restrict__ vh,
+    float* __restrict__ aabb_partial, uint32_t n) {
+  uint32_t scene;
+  uint32_t i = particle_of(blocks, dps, scene);
+  bool valid = i != 0xFFFFFFFFu;
+  DevParams* dp = dps + scene;
   float4 outp = make_float4(0.f, 0.f, 0.f, 0.f);
   uint32_t capped = 0;
   if (valid) {
@@ -179,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_advect(
       rpos = np;
       rvel = nv;
       float elapsed = time_to_go;
-      Hit hh = detect(p, np, mesh);
+      Hit hh = detect(p, np, mesh, dp);
       happened = hh.hit;
       if (happened) {
         // respond (collisions.cl:91-104)
@@ -204,48 +198,28 @@ __global__ __launch_bounds__(kBlock) void k_advect(
       vh[o] = make_float4(rvel.x, rvel.y, rvel.z, 0.f);
     }
   }
-  // Per-block bounding box of the new positions (NaN skipped) and the count
-  // of particles that reached the bounce cap.
-  __shared__ float red[6][kBlock / 64];
-  __shared__ uint32_t cap_red[kBlock / 64];
-  const float inf = __builtin_huge_valf();
-  float bx[6];
-  bool vx = valid && outp.x == outp.x, vy = valid && outp.y == outp.y,
-       vz = valid && outp.z == outp.z;
-  bx[0] = wave_min(vx ? outp.x : inf);
-  bx[1] = wave_min(vy ? outp.y : inf);
-  bx[2] = wave_min(vz ? outp.z : inf);
-  bx[3] = wave_max(vx ? outp.x : -inf);
-  bx[4] = wave_max(vy ? outp.y : -inf);
-  bx[5] = wave_max(vz ? outp.z : -inf);
+  // Work-group bounding box of the new positions for the next substep's grid
+  // (the work-group lies in one scene), and the count of particles that
+  // reached the bounce cap.
+  block_aabb_partial(outp, valid, aabb_partial);
+  __shared__ uint32_t cap_red[kMaxBlock / 64];
   uint64_t cb = __builtin_amdgcn_ballot_w64(capped != 0);
-  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  if (l == 0) {
-    for (int k = 0; k < 6; ++k) red[k][w] = bx[k];
-    cap_red[w] = (uint32_t)__builtin_popcountll(cb);
-  }
+  if ((threadIdx.x & 63) == 0) cap_red[threadIdx.x >> 6] = (uint32_t)__builtin_popcountll(cb);
   __syncthreads();
-  if (threadIdx.x < 6) {
-    int k = threadIdx.x;
-    float r = red[k][0];
-    for (int q = 1; q < kBlock / 64; ++q) r = (k < 3) ? fminf(r, red[k][q]) : fmaxf(r, red[k][q]);
-    aabb_partial[blockIdx.x * 6 + k] = r;
-  }
-  if (threadIdx.x == 6) {
+  if (threadIdx.x == 0) {
     uint32_t c = 0;
-    for (int q = 0; q < kBlock / 64; ++q) c += cap_red[q];
+    for (uint32_t q = 0; q < (blockDim.x + 63) / 64; ++q) c += cap_red[q];
     if (c) atomicAdd(&dp->capped, c);
   }
 }
 
 void launch_advect(clsph_state* s) {
   if (s->n == 0) return;
-  Mesh m{s->face_normals, s->vertices, s->indices, s->face_count, s->tgrid, s->tgrid_start,
-         s->tgrid_tris};
-  uint32_t blocks = (s->n + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_advect, dim3(blocks), dim3(kBlock), 0, s->ctx->stream, s->pos_s,
-                     s->vh_s, s->acc_s, s->slot, m, s->dp, s->pos, s->vel, s->vh,
-                     s->aabb_partial, s->n);
+  Mesh m{s->face_normals, s->vertices, s->indices, s->tgrid_start, s->tgrid_tris};
+  hipLaunchKernelGGL(k_advect, dim3(s->nblocks_of(kStageAdvect)), dim3(s->bs_of(kStageAdvect)), 0,
+                     s->ctx->stream, s->pos_s, s->vh_s, s->acc_s, s->slot, m,
+                     s->table_of(kStageAdvect), s->dps, s->pos, s->vel, s->vh, s->aabb_partial,
+                     s->n);
 }
 
 }  // namespace clsph

@@ -1,5 +1,6 @@
 // C ABI of the clsph HIP pipeline (include/clsph_hip.h).
 #include <math.h>
+#include <stddef.h>
 #include <string.h>
 
 #include <algorithm>
@@ -46,6 +47,12 @@ int dalloc(clsph_ctx* ctx, T** p, size_t count) {
   return CLSPH_OK;
 }
 
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
 int check_launch(clsph_ctx* ctx) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
@@ -53,29 +60,231 @@ int check_launch(clsph_ctx* ctx) {
   return CLSPH_OK;
 }
 
+int read_dps(clsph_state* st, std::vector<DevParams>& out, BatchParams* bp = nullptr) {
+  out.resize(st->n_scenes);
+  hipStream_t s = st->ctx->stream;
+  HIPCK(st->ctx, hipMemcpyAsync(out.data(), st->dps, sizeof(DevParams) * st->n_scenes,
+                                hipMemcpyDeviceToHost, s));
+  if (bp) HIPCK(st->ctx, hipMemcpyAsync(bp, st->bp, sizeof(BatchParams), hipMemcpyDeviceToHost, s));
+  HIPCK(st->ctx, hipStreamSynchronize(s));
+  return CLSPH_OK;
+}
+
 int device_errors(clsph_state* st) {
-  uint32_t err = 0;
-  HIPCK(st->ctx, hipMemcpyAsync(&err, &st->dp->err, sizeof(err), hipMemcpyDeviceToHost,
-                                st->ctx->stream));
-  HIPCK(st->ctx, hipStreamSynchronize(st->ctx->stream));
+  std::vector<DevParams> d;
+  BatchParams b;
+  int rc = read_dps(st, d, &b);
+  if (rc) return rc;
+  uint32_t err = b.err;
+  for (const DevParams& x : d) err |= x.err;
   if (err & kErrGridDim)
     return fail(st->ctx, CLSPH_EGRID,
                 "grid dimension >= 1024 cells on an axis (particles spread too far)");
   if (err & kErrTableCap)
-    return fail(st->ctx, CLSPH_ETABLE, "grid_cell_count exceeds the cell table capacity");
+    return fail(st->ctx, CLSPH_ETABLE, "grid cells exceed the cell table capacity");
+  if (err & kErrFineKey)
+    return fail(st->ctx, CLSPH_ETABLE, "grid_cell_count reached 2^29 cells");
   return CLSPH_OK;
 }
 
 void free_state_buffers(clsph_state* st) {
-  void* bufs[] = {st->tgrid_start, st->tgrid_tris,
-                  st->pos,  st->vel,   st->vh,    st->pos_s, st->vel_s, st->vh_s,
-                  st->acc_s, st->rho_s, st->prs_s, st->fa, st->fb, st->key_a, st->val_a,
-                  st->key_b, st->val_b, st->hist,  st->digit_base, st->table,
-                  st->aabb_partial, st->face_normals, st->vertices, st->indices, st->dp,
-                  st->nbr, st->nbr_count, st->slot};
-  for (void* b : bufs)
-    if (b) (void)hipFree(b);
+  dfree(st->pos), dfree(st->vel), dfree(st->vh);
+  dfree(st->pos_s), dfree(st->vel_s), dfree(st->vh_s), dfree(st->acc_s);
+  dfree(st->rho_s), dfree(st->prs_s), dfree(st->fa), dfree(st->fb);
+  dfree(st->nbr), dfree(st->nbr_count);
+  dfree(st->key_a), dfree(st->val_a), dfree(st->key_b), dfree(st->val_b);
+  dfree(st->hist), dfree(st->digit_base), dfree(st->table), dfree(st->slot);
+  dfree(st->aabb_partial);
+  dfree(st->face_normals), dfree(st->vertices), dfree(st->indices);
+  dfree(st->tgrid_start), dfree(st->tgrid_tris);
+  dfree(st->dps), dfree(st->bp);
+  for (auto& b : st->blocks) dfree(b);
   if (st->graph) (void)hipGraphExecDestroy(st->graph);
+  st->graph = nullptr;
+}
+
+// Work-group tables for every block size, and each scene's range in the
+// advection table (where the bounding-box partials live).
+int build_blocks(clsph_state* st) {
+  for (int b = 0; b < kNumBlockSizes; ++b) {
+    uint32_t bs = (uint32_t)kBlockSizes[b];
+    std::vector<BlockDesc> t;
+    for (uint32_t s = 0; s < st->n_scenes; ++s) {
+      if (b == st->stage_bs[kStageAdvect]) st->host_dps[s].block_begin = (uint32_t)t.size();
+      for (uint32_t f = 0; f < st->scene_n[s]; f += bs) t.push_back({s, st->scene_begin[s] + f});
+      if (b == st->stage_bs[kStageAdvect]) st->host_dps[s].block_end = (uint32_t)t.size();
+    }
+    dfree(st->blocks[b]);
+    int rc = dalloc(st->ctx, &st->blocks[b], t.size());
+    if (rc) return rc;
+    if (!t.empty())
+      HIPCK(st->ctx, hipMemcpy(st->blocks[b], t.data(), sizeof(BlockDesc) * t.size(),
+                               hipMemcpyHostToDevice));
+    st->nblocks[b] = (uint32_t)t.size();
+  }
+  st->graph_dirty = true;
+  return CLSPH_OK;
+}
+
+int upload_params(clsph_state* st) {
+  for (uint32_t s = 0; s < st->n_scenes; ++s) {
+    st->host_dps[s].begin = st->scene_begin[s];
+    st->host_dps[s].n = st->scene_n[s];
+    st->host_dps[s].err = 0;
+    st->host_dps[s].capped = 0;
+  }
+  BatchParams b{};
+  b.n_scenes = st->n_scenes;
+  b.n = st->n;
+  b.table_cap = st->table_cap;
+  HIPCK(st->ctx, hipMemcpyAsync(st->dps, st->host_dps.data(), sizeof(DevParams) * st->n_scenes,
+                                hipMemcpyHostToDevice, st->ctx->stream));
+  HIPCK(st->ctx, hipMemcpyAsync(st->bp, &b, sizeof(b), hipMemcpyHostToDevice, st->ctx->stream));
+  HIPCK(st->ctx, hipStreamSynchronize(st->ctx->stream));
+  return CLSPH_OK;
+}
+
+// Uniform grid over one scene's mesh for collision queries: cell c lists
+// every triangle whose bounding box, padded by a rounding margin, meets it.
+// Face indices are global (face_begin added); cell starts are global offsets.
+void build_tri_grid(const clsph_state::Mesh& m, uint32_t face_begin, float cell_size,
+                    TriGrid& g, std::vector<uint32_t>& starts, std::vector<uint32_t>& tris) {
+  g = TriGrid{};
+  uint32_t faces = (uint32_t)(m.indices.size() / 3);
+  if (faces == 0 || cell_size < 0.f) return;
+  const float* verts = m.vertices.data();
+  const uint32_t* idx = m.indices.data();
+  float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+  double ext_sum = 0.0;
+  std::vector<float> box(6ull * faces);
+  for (uint32_t f = 0; f < faces; ++f) {
+    float* tb = &box[6ull * f];
+    for (int a = 0; a < 3; ++a) tb[a] = 1e30f, tb[3 + a] = -1e30f;
+    for (int k = 0; k < 3; ++k)
+      for (int a = 0; a < 3; ++a) {
+        float v = verts[3 * idx[3 * f + k] + a];
+        tb[a] = std::min(tb[a], v);
+        tb[3 + a] = std::max(tb[3 + a], v);
+      }
+    float e = 0.f;
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], tb[a]);
+      hi[a] = std::max(hi[a], tb[3 + a]);
+      e = std::max(e, tb[3 + a] - tb[a]);
+    }
+    ext_sum += e;
+  }
+  if (!(cell_size > 0.f)) cell_size = (float)(2.0 * ext_sum / faces);
+  float span = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+  if (!(cell_size > 0.f)) cell_size = span > 0.f ? span : 1.f;
+  // Rounding margin: hit points are computed in FP32 and may stray from the
+  // exact triangle by a few ulps of the coordinates involved.
+  float mag = 0.f;
+  for (int a = 0; a < 3; ++a) mag = std::max(mag, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
+  float pad = 1e-5f * (mag + cell_size) + 1e-6f;
+  uint32_t dims[3];
+  for (;;) {
+    uint64_t total = 1;
+    for (int a = 0; a < 3; ++a) {
+      double ext = (double)(hi[a] - lo[a]) + 2.0 * (pad + cell_size);
+      dims[a] = (uint32_t)std::max(1.0, std::ceil(ext / (double)cell_size));
+      total *= dims[a];
+    }
+    if (total <= (1ull << 25)) break;
+    cell_size *= 1.5f;
+  }
+  float origin[3];
+  for (int a = 0; a < 3; ++a) origin[a] = lo[a] - pad - cell_size;
+  float inv = 1.f / cell_size;
+  auto cell_of = [&](float v, int a) {
+    float c = std::floor((v - origin[a]) * inv);
+    c = std::max(0.f, std::min(c, (float)(dims[a] - 1)));
+    return (uint32_t)c;
+  };
+  uint64_t ncells = (uint64_t)dims[0] * dims[1] * dims[2];
+  std::vector<uint32_t> count(ncells, 0);
+  std::vector<uint32_t> range(6ull * faces);
+  for (uint32_t f = 0; f < faces; ++f) {
+    for (int a = 0; a < 3; ++a) {
+      range[6 * f + a] = cell_of(box[6ull * f + a] - pad, a);
+      range[6 * f + 3 + a] = cell_of(box[6ull * f + 3 + a] + pad, a);
+    }
+    for (uint32_t z = range[6 * f + 2]; z <= range[6 * f + 5]; ++z)
+      for (uint32_t y = range[6 * f + 1]; y <= range[6 * f + 4]; ++y)
+        for (uint32_t x = range[6 * f + 0]; x <= range[6 * f + 3]; ++x)
+          count[((uint64_t)z * dims[1] + y) * dims[0] + x]++;
+  }
+  g.cell_base = (uint32_t)starts.size();
+  uint32_t run = (uint32_t)tris.size();
+  std::vector<uint32_t> fill(ncells);
+  for (uint64_t c = 0; c < ncells; ++c) {
+    starts.push_back(run);
+    fill[c] = run;
+    run += count[c];
+  }
+  starts.push_back(run);
+  tris.resize(run);
+  for (uint32_t f = 0; f < faces; ++f)
+    for (uint32_t z = range[6 * f + 2]; z <= range[6 * f + 5]; ++z)
+      for (uint32_t y = range[6 * f + 1]; y <= range[6 * f + 4]; ++y)
+        for (uint32_t x = range[6 * f + 0]; x <= range[6 * f + 3]; ++x)
+          tris[fill[((uint64_t)z * dims[1] + y) * dims[0] + x]++] = face_begin + f;
+  for (int a = 0; a < 3; ++a) {
+    g.origin[a] = origin[a];
+    g.dims[a] = dims[a];
+  }
+  g.inv_cell = inv;
+  g.enabled = 1;
+}
+
+// Concatenates every scene's mesh (and triangle grid) into the device arrays.
+int rebuild_meshes(clsph_state* st) {
+  HIPCK(st->ctx, hipStreamSynchronize(st->ctx->stream));
+  std::vector<float> normals, verts;
+  std::vector<uint32_t> idx, starts, tris;
+  for (uint32_t s = 0; s < st->n_scenes; ++s) {
+    const clsph_state::Mesh& m = st->meshes[s];
+    uint32_t vbase = (uint32_t)(verts.size() / 3);
+    uint32_t fbase = (uint32_t)(idx.size() / 3);
+    normals.insert(normals.end(), m.normals.begin(), m.normals.end());
+    verts.insert(verts.end(), m.vertices.begin(), m.vertices.end());
+    for (uint32_t k : m.indices) idx.push_back(vbase + k);
+    st->host_dps[s].face_begin = fbase;
+    st->host_dps[s].face_count = (uint32_t)(m.indices.size() / 3);
+    build_tri_grid(m, fbase, m.grid_cell, st->host_dps[s].tgrid, starts, tris);
+  }
+  dfree(st->face_normals), dfree(st->vertices), dfree(st->indices);
+  dfree(st->tgrid_start), dfree(st->tgrid_tris);
+  int rc = dalloc(st->ctx, &st->face_normals, normals.size());
+  if (!rc) rc = dalloc(st->ctx, &st->vertices, verts.size());
+  if (!rc) rc = dalloc(st->ctx, &st->indices, idx.size());
+  if (!rc) rc = dalloc(st->ctx, &st->tgrid_start, starts.size());
+  if (!rc) rc = dalloc(st->ctx, &st->tgrid_tris, tris.size());
+  if (rc) return rc;
+  auto up = [&](void* d, const void* h, size_t bytes) -> int {
+    if (bytes) HIPCK(st->ctx, hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+    return CLSPH_OK;
+  };
+  rc = up(st->face_normals, normals.data(), 4 * normals.size());
+  if (!rc) rc = up(st->vertices, verts.data(), 4 * verts.size());
+  if (!rc) rc = up(st->indices, idx.data(), 4 * idx.size());
+  if (!rc) rc = up(st->tgrid_start, starts.data(), 4 * starts.size());
+  if (!rc) rc = up(st->tgrid_tris, tris.data(), 4 * tris.size());
+  if (rc) return rc;
+  st->graph_dirty = true;
+  // Only the mesh fields change; the rest of the device parameters is kept.
+  for (uint32_t s = 0; s < st->n_scenes; ++s) {
+    const DevParams& h = st->host_dps[s];
+    size_t off = offsetof(DevParams, face_begin);
+    HIPCK(st->ctx, hipMemcpy(reinterpret_cast<char*>(st->dps + s) + off,
+                             reinterpret_cast<const char*>(&h) + off,
+                             offsetof(DevParams, block_begin) - off, hipMemcpyHostToDevice));
+    off = offsetof(DevParams, tgrid);
+    HIPCK(st->ctx, hipMemcpy(reinterpret_cast<char*>(st->dps + s) + off,
+                             reinterpret_cast<const char*>(&h) + off, sizeof(TriGrid),
+                             hipMemcpyHostToDevice));
+  }
+  return CLSPH_OK;
 }
 
 // Launches the kernels of one whole substep on the context stream.
@@ -92,13 +301,13 @@ void enqueue_step(clsph_state* st) {
 
 int run_stage(clsph_state* st, int stage) {
   switch (stage) {
-    case 0:
+    case kStageGrid:
       launch_grid_params(st);
       break;
-    case 1:
+    case kStageHash:
       launch_hash(st);
       break;
-    case 2:
+    case kStageSort:
       // sorting also builds the tables and the canonical slots, so that a
       // slot is always current once particles have been reordered
       launch_sort(st);
@@ -106,16 +315,16 @@ int run_stage(clsph_state* st, int stage) {
       launch_cell_table(st);
       st->fields_canonical = false;
       break;
-    case 3:
+    case kStageTable:
       launch_cell_table(st);
       break;
-    case 4:
+    case kStageDensity:
       launch_density(st);
       break;
-    case 5:
+    case kStageForces:
       launch_forces(st);
       break;
-    case 6:
+    case kStageAdvect:
       launch_advect(st);
       st->acc_zero = true;
       break;
@@ -126,112 +335,260 @@ int run_stage(clsph_state* st, int stage) {
   return check_launch(st->ctx);
 }
 
-// Uniform grid over the mesh for collision queries: cell c lists every
-// triangle whose bounding box, padded by a rounding margin, meets it. Built
-// on the host once per mesh (the mesh is static).
-int build_tri_grid(clsph_state* st, const float* verts, const uint32_t* idx, uint32_t faces,
-                   float cell_size) {
-  clsph_ctx* ctx = st->ctx;
-  if (st->tgrid_start) (void)hipFree(st->tgrid_start);
-  if (st->tgrid_tris) (void)hipFree(st->tgrid_tris);
-  st->tgrid_start = nullptr;
-  st->tgrid_tris = nullptr;
-  st->tgrid.enabled = 0;
-  if (faces == 0 || cell_size < 0.f) return CLSPH_OK;
-  float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
-  double ext_sum = 0.0;
-  for (uint32_t f = 0; f < faces; ++f) {
-    float tlo[3] = {1e30f, 1e30f, 1e30f}, thi[3] = {-1e30f, -1e30f, -1e30f};
-    for (int k = 0; k < 3; ++k)
-      for (int a = 0; a < 3; ++a) {
-        float v = verts[3 * idx[3 * f + k] + a];
-        tlo[a] = std::min(tlo[a], v);
-        thi[a] = std::max(thi[a], v);
-      }
-    float e = 0.f;
-    for (int a = 0; a < 3; ++a) {
-      lo[a] = std::min(lo[a], tlo[a]);
-      hi[a] = std::max(hi[a], thi[a]);
-      e = std::max(e, thi[a] - tlo[a]);
-    }
-    ext_sum += e;
-  }
-  if (!(cell_size > 0.f)) cell_size = (float)(2.0 * ext_sum / faces);
-  float span = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
-  if (!(cell_size > 0.f)) cell_size = span > 0.f ? span : 1.f;
-  // Rounding margin: hit points are computed in FP32 and may stray from the
-  // exact triangle by a few ulps of the coordinates involved.
-  float mag = 0.f;
-  for (int a = 0; a < 3; ++a) mag = std::max(mag, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
-  float pad = 1e-5f * (mag + cell_size) + 1e-6f;
-  uint32_t dims[3];
-  for (;;) {
-    uint64_t total = 1;
-    for (int a = 0; a < 3; ++a) {
-      float ext = (hi[a] - lo[a]) + 2.f * (pad + cell_size);
-      dims[a] = (uint32_t)std::max(1.0, std::ceil((double)ext / (double)cell_size));
-      total *= dims[a];
-    }
-    if (total <= (1ull << 25)) break;
-    cell_size *= 1.5f;
-  }
-  float origin[3];
-  for (int a = 0; a < 3; ++a) origin[a] = lo[a] - pad - cell_size;
-  float inv = 1.f / cell_size;
-  auto cell_of = [&](float v, int a) {
-    float c = std::floor((v - origin[a]) * inv);
-    c = std::max(0.f, std::min(c, (float)(dims[a] - 1)));
-    return (uint32_t)c;
-  };
-  uint64_t ncells = (uint64_t)dims[0] * dims[1] * dims[2];
-  std::vector<uint32_t> count(ncells + 1, 0);
-  std::vector<uint32_t> range(6ull * faces);
-  for (uint32_t f = 0; f < faces; ++f) {
-    float tlo[3] = {1e30f, 1e30f, 1e30f}, thi[3] = {-1e30f, -1e30f, -1e30f};
-    for (int k = 0; k < 3; ++k)
-      for (int a = 0; a < 3; ++a) {
-        float v = verts[3 * idx[3 * f + k] + a];
-        tlo[a] = std::min(tlo[a], v);
-        thi[a] = std::max(thi[a], v);
-      }
-    for (int a = 0; a < 3; ++a) {
-      range[6 * f + a] = cell_of(tlo[a] - pad, a);
-      range[6 * f + 3 + a] = cell_of(thi[a] + pad, a);
-    }
-    for (uint32_t z = range[6 * f + 2]; z <= range[6 * f + 5]; ++z)
-      for (uint32_t y = range[6 * f + 1]; y <= range[6 * f + 4]; ++y)
-        for (uint32_t x = range[6 * f + 0]; x <= range[6 * f + 3]; ++x)
-          count[((uint64_t)z * dims[1] + y) * dims[0] + x]++;
-  }
-  std::vector<uint32_t> start(ncells + 1, 0);
-  for (uint64_t c = 0; c < ncells; ++c) start[c + 1] = start[c] + count[c];
-  std::vector<uint32_t> tris(std::max<uint32_t>(start[ncells], 1));
-  std::vector<uint32_t> fill(start.begin(), start.end() - 1);
-  for (uint32_t f = 0; f < faces; ++f)
-    for (uint32_t z = range[6 * f + 2]; z <= range[6 * f + 5]; ++z)
-      for (uint32_t y = range[6 * f + 1]; y <= range[6 * f + 4]; ++y)
-        for (uint32_t x = range[6 * f + 0]; x <= range[6 * f + 3]; ++x)
-          tris[fill[((uint64_t)z * dims[1] + y) * dims[0] + x]++] = f;
-  int rc = dalloc(ctx, &st->tgrid_start, ncells + 1);
-  if (!rc) rc = dalloc(ctx, &st->tgrid_tris, tris.size());
-  if (rc) return rc;
-  HIPCK(ctx, hipMemcpy(st->tgrid_start, start.data(), 4 * (ncells + 1), hipMemcpyHostToDevice));
-  HIPCK(ctx, hipMemcpy(st->tgrid_tris, tris.data(), 4 * tris.size(), hipMemcpyHostToDevice));
-  for (int a = 0; a < 3; ++a) {
-    st->tgrid.origin[a] = origin[a];
-    st->tgrid.dims[a] = dims[a];
-  }
-  st->tgrid.inv_cell = inv;
-  st->tgrid.enabled = 1;
-  st->graph_dirty = true;
-  return CLSPH_OK;
-}
-
 uint64_t splitmix64(uint64_t x) {
   x += 0x9E3779B97F4A7C15ull;
   x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
   x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
   return x ^ (x >> 31);
+}
+
+int alloc_state(clsph_ctx* ctx, uint32_t n_scenes, const uint32_t* caps, bool fixed,
+                uint32_t table_cap, clsph_state** out) {
+  if (!ctx || !out || n_scenes == 0 || !caps)
+    return fail(ctx, CLSPH_EINVAL, "state allocation: bad argument");
+  *out = nullptr;
+  HIPCK(ctx, hipSetDevice(ctx->device));
+  clsph_state* st = new clsph_state();
+  st->ctx = ctx;
+  st->n_scenes = n_scenes;
+  st->fixed_counts = fixed;
+  uint64_t total = 0, blocks64 = 0;
+  for (uint32_t s = 0; s < n_scenes; ++s) {
+    st->scene_begin.push_back((uint32_t)total);
+    st->scene_cap.push_back(caps[s]);
+    st->scene_n.push_back(fixed ? caps[s] : 0);
+    total += caps[s];
+    blocks64 += (caps[s] + 63) / 64;
+  }
+  if (total > 0x7FFFFFFFull) {
+    delete st;
+    return fail(ctx, CLSPH_EINVAL, "too many particles for one state");
+  }
+  st->n_cap = (uint32_t)total;
+  st->n = fixed ? (uint32_t)total : 0;
+  uint64_t cells = table_cap ? table_cap : std::max<uint64_t>(1u << 20, 8ull * total);
+  if (cells > (1ull << 28)) {
+    delete st;
+    return fail(ctx, CLSPH_EINVAL, "table_cap above 2^28 cells is not supported");
+  }
+  st->table_cap = (uint32_t)(8ull * cells);
+  uint32_t n_cap = st->n_cap;
+  st->n_tiles_cap = sort_tiles_for(n_cap ? n_cap : 1);
+  int rc = CLSPH_OK;
+  float4** f4[] = {&st->pos, &st->vel, &st->vh, &st->pos_s, &st->vel_s, &st->vh_s,
+                   &st->acc_s, &st->fa, &st->fb};
+  for (auto p : f4)
+    if (rc == CLSPH_OK) rc = dalloc(ctx, p, n_cap);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->rho_s, n_cap);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->prs_s, n_cap);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->nbr, (size_t)(kMaxNeighbours / 4) * n_cap);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->nbr_count, n_cap);
+  uint32_t** u4[] = {&st->key_a, &st->val_a, &st->key_b, &st->val_b, &st->slot};
+  for (auto p : u4)
+    if (rc == CLSPH_OK) rc = dalloc(ctx, p, n_cap);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->hist, (size_t)kMaxBins * st->n_tiles_cap);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->digit_base, (size_t)kMaxBins);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->table, (size_t)st->table_cap + 1);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->aabb_partial, (size_t)(blocks64 + 1) * 6);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->dps, n_scenes);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->bp, 1);
+  if (rc != CLSPH_OK) {
+    free_state_buffers(st);
+    delete st;
+    return rc;
+  }
+  st->host_dps.assign(n_scenes, DevParams{});
+  for (auto& d : st->host_dps) d.max_bounces = 64;
+  st->meshes.resize(n_scenes);
+  if (!rc) rc = build_blocks(st);
+  if (!rc) rc = upload_params(st);
+  if (!rc) rc = rebuild_meshes(st);
+  if (rc) {
+    free_state_buffers(st);
+    delete st;
+    return rc;
+  }
+  *out = st;
+  return CLSPH_OK;
+}
+
+int set_params(clsph_state* st, uint32_t scene, const clsph_simulation_parameters* params,
+               const clsph_precomputed_kernel_values* terms, uint32_t max_bounces) {
+  if (!st || !params || !terms) return fail(st ? st->ctx : nullptr, CLSPH_EINVAL, "NULL argument");
+  if (scene >= st->n_scenes) return fail(st->ctx, CLSPH_EINVAL, "scene index out of range");
+  if (!(params->h > 0.f)) return fail(st->ctx, CLSPH_EINVAL, "h must be positive");
+  DevParams& d = st->host_dps[scene];
+  d.p = *params;
+  d.t = *terms;
+  d.max_bounces = max_bounces ? max_bounces : 64;
+  return upload_params(st);
+}
+
+int upload(clsph_state* st, uint32_t scene, const clsph_particle* p, uint32_t n) {
+  if (!st || (!p && n)) return fail(st ? st->ctx : nullptr, CLSPH_EINVAL, "NULL argument");
+  if (scene >= st->n_scenes) return fail(st->ctx, CLSPH_EINVAL, "scene index out of range");
+  if (n > st->scene_cap[scene]) return fail(st->ctx, CLSPH_EINVAL, "more particles than the scene holds");
+  if (st->fixed_counts && n != st->scene_n[scene])
+    return fail(st->ctx, CLSPH_EINVAL, "a batch scene takes exactly its allocated count");
+  std::vector<float4> pos(n), vel(n), vh(n), acc(n);
+  std::vector<float> rho(n), prs(n);
+  std::vector<uint32_t> key(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* a = p[i].position.s;
+    const float* b = p[i].velocity.s;
+    const float* c = p[i].intermediate_velocity.s;
+    const float* d = p[i].acceleration.s;
+    pos[i] = make_float4(a[0], a[1], a[2], 0.f);
+    vel[i] = make_float4(b[0], b[1], b[2], 0.f);
+    vh[i] = make_float4(c[0], c[1], c[2], 0.f);
+    acc[i] = make_float4(d[0], d[1], d[2], 0.f);
+    rho[i] = p[i].density;
+    prs[i] = p[i].pressure;
+    key[i] = p[i].grid_index;
+  }
+  uint32_t b0 = st->scene_begin[scene];
+  hipStream_t s = st->ctx->stream;
+  auto put = [&](void* dst, const void* src, size_t bytes) -> int {
+    if (bytes) HIPCK(st->ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return CLSPH_OK;
+  };
+  int rc = put(st->pos + b0, pos.data(), sizeof(float4) * n);
+  if (!rc) rc = put(st->vel + b0, vel.data(), sizeof(float4) * n);
+  if (!rc) rc = put(st->vh + b0, vh.data(), sizeof(float4) * n);
+  if (!rc) rc = put(st->acc_s + b0, acc.data(), sizeof(float4) * n);
+  if (!rc) rc = put(st->rho_s + b0, rho.data(), sizeof(float) * n);
+  if (!rc) rc = put(st->prs_s + b0, prs.data(), sizeof(float) * n);
+  if (!rc) rc = put(st->key_b + b0, key.data(), sizeof(uint32_t) * n);
+  if (rc) return rc;
+  if (n != st->scene_n[scene]) {
+    st->scene_n[scene] = n;
+    st->n = n;  // single-scene states only (batches keep their counts)
+    rc = build_blocks(st);
+    if (!rc) rc = upload_params(st);
+    if (rc) return rc;
+  }
+  launch_aabb(st);
+  st->acc_zero = false;
+  st->fields_canonical = true;
+  st->stage = -1;
+  rc = check_launch(st->ctx);
+  if (rc) return rc;
+  HIPCK(st->ctx, hipStreamSynchronize(s));
+  return CLSPH_OK;
+}
+
+// Copies scene `scene` to host particles. Fields computed in half-cell order
+// go to each particle's canonical slot; `sorted` returns the reference's
+// sorted buffer (stable by cell id) of the last substep instead of the state.
+int download(clsph_state* st, uint32_t scene, clsph_particle* out, uint32_t n, bool sorted) {
+  if (!st || (!out && n)) return fail(st ? st->ctx : nullptr, CLSPH_EINVAL, "NULL argument");
+  if (scene >= st->n_scenes) return fail(st->ctx, CLSPH_EINVAL, "scene index out of range");
+  uint32_t sn = st->scene_n[scene];
+  if (n > sn) return fail(st->ctx, CLSPH_EINVAL, "asked for more particles than are held");
+  if (n == 0) return device_errors(st);
+  if (n != sn && !st->fields_canonical)
+    return fail(st->ctx, CLSPH_EINVAL, "partial downloads need the whole particle count");
+  uint32_t b0 = st->scene_begin[scene];
+  uint32_t cnt = n;
+  std::vector<float4> pos(cnt), vel(cnt), vh(cnt), acc(cnt);
+  std::vector<float> rho(cnt), prs(cnt);
+  std::vector<uint32_t> key(cnt), slot(cnt);
+  hipStream_t s = st->ctx->stream;
+  const float4* P = (sorted ? st->pos_s : st->pos) + b0;
+  const float4* V = (sorted ? st->vel_s : st->vel) + b0;
+  const float4* H = (sorted ? st->vh_s : st->vh) + b0;
+  auto get = [&](void* dst, const void* src, size_t bytes) -> int {
+    HIPCK(st->ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    return CLSPH_OK;
+  };
+  int rc = get(pos.data(), P, sizeof(float4) * cnt);
+  if (!rc) rc = get(vel.data(), V, sizeof(float4) * cnt);
+  if (!rc) rc = get(vh.data(), H, sizeof(float4) * cnt);
+  bool want_acc = sorted || !st->acc_zero;
+  if (!rc && want_acc) rc = get(acc.data(), st->acc_s + b0, sizeof(float4) * cnt);
+  if (!rc) rc = get(rho.data(), st->rho_s + b0, sizeof(float) * cnt);
+  if (!rc) rc = get(prs.data(), st->prs_s + b0, sizeof(float) * cnt);
+  if (!rc) rc = get(key.data(), st->key_b + b0, sizeof(uint32_t) * cnt);
+  bool reorder = !st->fields_canonical;
+  if (!rc && reorder) rc = get(slot.data(), st->slot + b0, sizeof(uint32_t) * cnt);
+  if (rc) return rc;
+  std::vector<DevParams> d;
+  rc = read_dps(st, d);  // also waits for the copies above
+  if (rc) return rc;
+  uint32_t kbase = reorder ? d[scene].key_base : 0;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    uint32_t o = reorder ? slot[k] - b0 : k;
+    if (o >= n) return fail(st->ctx, CLSPH_EDEVICE, "canonical slots are not a permutation");
+    clsph_particle& q = out[o];
+    memset(&q, 0, sizeof(q));
+    q.density = rho[k];
+    q.pressure = prs[k];
+    q.grid_index = reorder ? (key[k] - kbase) >> 3 : key[k];
+    if (want_acc && (sorted || !reorder)) {
+      q.acceleration.s[0] = acc[k].x, q.acceleration.s[1] = acc[k].y, q.acceleration.s[2] = acc[k].z;
+    }
+  }
+  for (uint32_t k = 0; k < cnt; ++k) {
+    uint32_t o = (sorted && reorder) ? slot[k] - b0 : k;
+    clsph_particle& q = out[o];
+    q.position.s[0] = pos[k].x, q.position.s[1] = pos[k].y, q.position.s[2] = pos[k].z;
+    q.velocity.s[0] = vel[k].x, q.velocity.s[1] = vel[k].y, q.velocity.s[2] = vel[k].z;
+    q.intermediate_velocity.s[0] = vh[k].x, q.intermediate_velocity.s[1] = vh[k].y,
+    q.intermediate_velocity.s[2] = vh[k].z;
+  }
+  return device_errors(st);
+}
+
+int scene_upload(clsph_state* st, uint32_t scene, const float* face_normals,
+                 const float* vertices, uint32_t vertex_count, const uint32_t* indices,
+                 uint32_t face_count) {
+  if (!st) return fail(nullptr, CLSPH_EINVAL, "NULL state");
+  if (scene >= st->n_scenes) return fail(st->ctx, CLSPH_EINVAL, "scene index out of range");
+  if (face_count && (!face_normals || !vertices || !indices))
+    return fail(st->ctx, CLSPH_EINVAL, "NULL mesh array");
+  for (uint64_t k = 0; k < 3ull * face_count; ++k)
+    if (indices[k] >= vertex_count)
+      return fail(st->ctx, CLSPH_EINVAL, "triangle index out of range of the vertex array");
+  clsph_state::Mesh& m = st->meshes[scene];
+  m.normals.assign(face_normals, face_normals + 3ull * face_count);
+  m.vertices.assign(vertices, vertices + 3ull * vertex_count);
+  m.indices.assign(indices, indices + 3ull * face_count);
+  m.grid_cell = face_count > 64 ? 0.f : -1.f;
+  return rebuild_meshes(st);
+}
+
+int read_u32(clsph_state* st, const uint32_t* src, uint32_t* dst, size_t count) {
+  if (count == 0) return CLSPH_OK;
+  HIPCK(st->ctx, hipMemcpyAsync(dst, src, sizeof(uint32_t) * count, hipMemcpyDeviceToHost,
+                                st->ctx->stream));
+  HIPCK(st->ctx, hipStreamSynchronize(st->ctx->stream));
+  return CLSPH_OK;
+}
+
+int ensure_graph(clsph_state* st) {
+  if (!st->graph_dirty && st->graph) return CLSPH_OK;
+  if (st->graph) {
+    (void)hipGraphExecDestroy(st->graph);
+    st->graph = nullptr;
+  }
+  hipStream_t s = st->ctx->stream;
+  hipGraph_t g = nullptr;
+  HIPCK(st->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  enqueue_step(st);
+  hipError_t le = hipGetLastError();
+  HIPCK(st->ctx, hipStreamEndCapture(s, &g));
+  if (le != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(st->ctx, CLSPH_EDEVICE, std::string("capture: ") + hipGetErrorString(le));
+  }
+  hipError_t ie = hipGraphInstantiate(&st->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess) {
+    st->graph = nullptr;
+    return fail(st->ctx, CLSPH_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(ie));
+  }
+  st->graph_dirty = false;
+  return CLSPH_OK;
 }
 
 }  // namespace
@@ -288,51 +645,12 @@ int clsph_sync(clsph_ctx* ctx) {
 }
 
 int clsph_state_alloc(clsph_ctx* ctx, uint32_t n_cap, uint32_t table_cap, clsph_state** out) {
-  if (!ctx || !out) return fail(ctx, CLSPH_EINVAL, "clsph_state_alloc: NULL argument");
-  *out = nullptr;
-  if (table_cap == 0) {
-    uint64_t c = 8ull * n_cap;
-    table_cap = (uint32_t)(c < (1u << 20) ? (1u << 20) : (c > (1u << 28) ? (1u << 28) : c));
-  }
-  if (table_cap > (1u << 29))
-    return fail(ctx, CLSPH_EINVAL, "table_cap above 2^29 cells is not supported");
-  HIPCK(ctx, hipSetDevice(ctx->device));
-  clsph_state* st = new clsph_state();
-  st->ctx = ctx;
-  st->n_cap = n_cap;
-  st->table_cap = table_cap;
-  st->n_tiles_cap = sort_tiles_for(n_cap ? n_cap : 1);
-  uint32_t nb = (n_cap + 255) / 256 + 1;
-  int rc = CLSPH_OK;
-  float4** f4[] = {&st->pos, &st->vel, &st->vh, &st->pos_s, &st->vel_s, &st->vh_s, &st->acc_s};
-  for (auto p : f4)
-    if (rc == CLSPH_OK) rc = dalloc(ctx, p, n_cap);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->rho_s, n_cap);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->prs_s, n_cap);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->fa, n_cap);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->fb, n_cap);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->nbr, (size_t)(kMaxNeighbours / 4) * n_cap);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->nbr_count, n_cap);
-  uint32_t** u4[] = {&st->key_a, &st->val_a, &st->key_b, &st->val_b};
-  for (auto p : u4)
-    if (rc == CLSPH_OK) rc = dalloc(ctx, p, n_cap);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->hist, (size_t)kMaxBins * st->n_tiles_cap);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->digit_base, 2 * (size_t)kMaxBins);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->table, 8ull * table_cap + 1);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->slot, n_cap);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->aabb_partial, (size_t)nb * 6);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->dp, 1);
-  if (rc != CLSPH_OK) {
-    free_state_buffers(st);
-    delete st;
-    return rc;
-  }
-  memset(&st->host_dp, 0, sizeof(st->host_dp));
-  st->host_dp.table_cap = table_cap;
-  st->host_dp.max_bounces = 64;
-  HIPCK(ctx, hipMemcpy(st->dp, &st->host_dp, sizeof(DevParams), hipMemcpyHostToDevice));
-  *out = st;
-  return CLSPH_OK;
+  return alloc_state(ctx, 1, &n_cap, false, table_cap, out);
+}
+
+int clsph_batch_alloc(clsph_ctx* ctx, uint32_t n_scenes, const uint32_t* counts,
+                      uint32_t table_cap, clsph_state** out) {
+  return alloc_state(ctx, n_scenes, counts, true, table_cap, out);
 }
 
 int clsph_state_free(clsph_state* st) {
@@ -346,199 +664,105 @@ int clsph_state_free(clsph_state* st) {
 
 int clsph_set_params(clsph_state* st, const clsph_simulation_parameters* params,
                      const clsph_precomputed_kernel_values* terms, uint32_t max_bounces) {
-  if (!st || !params || !terms) return fail(st ? st->ctx : nullptr, CLSPH_EINVAL, "NULL argument");
-  if (!(params->h > 0.f)) return fail(st->ctx, CLSPH_EINVAL, "h must be positive");
-  st->host_dp.p = *params;
-  st->host_dp.t = *terms;
-  st->host_dp.n = st->n;
-  st->host_dp.table_cap = st->table_cap;
-  st->host_dp.max_bounces = max_bounces ? max_bounces : 64;
-  st->host_dp.err = 0;
-  st->host_dp.capped = 0;
-  HIPCK(st->ctx, hipMemcpyAsync(st->dp, &st->host_dp, sizeof(DevParams), hipMemcpyHostToDevice,
-                                st->ctx->stream));
-  HIPCK(st->ctx, hipStreamSynchronize(st->ctx->stream));
-  return CLSPH_OK;
+  return set_params(st, 0, params, terms, max_bounces);
+}
+
+int clsph_batch_set_params(clsph_state* st, uint32_t scene,
+                           const clsph_simulation_parameters* params,
+                           const clsph_precomputed_kernel_values* terms, uint32_t max_bounces) {
+  return set_params(st, scene, params, terms, max_bounces);
 }
 
 int clsph_upload_aos(clsph_state* st, const clsph_particle* p, uint32_t n) {
-  if (!st || (!p && n)) return fail(st ? st->ctx : nullptr, CLSPH_EINVAL, "NULL argument");
-  if (n > st->n_cap) return fail(st->ctx, CLSPH_EINVAL, "more particles than the state holds");
-  std::vector<float4> pos(n), vel(n), vh(n), acc(n);
-  std::vector<float> rho(n), prs(n);
-  std::vector<uint32_t> key(n);
-  for (uint32_t i = 0; i < n; ++i) {
-    const float* a = p[i].position.s;
-    const float* b = p[i].velocity.s;
-    const float* c = p[i].intermediate_velocity.s;
-    const float* d = p[i].acceleration.s;
-    pos[i] = make_float4(a[0], a[1], a[2], 0.f);
-    vel[i] = make_float4(b[0], b[1], b[2], 0.f);
-    vh[i] = make_float4(c[0], c[1], c[2], 0.f);
-    acc[i] = make_float4(d[0], d[1], d[2], 0.f);
-    rho[i] = p[i].density;
-    prs[i] = p[i].pressure;
-    key[i] = p[i].grid_index;
-  }
-  hipStream_t s = st->ctx->stream;
-  HIPCK(st->ctx, hipMemcpyAsync(st->pos, pos.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s));
-  HIPCK(st->ctx, hipMemcpyAsync(st->vel, vel.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s));
-  HIPCK(st->ctx, hipMemcpyAsync(st->vh, vh.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s));
-  HIPCK(st->ctx, hipMemcpyAsync(st->acc_s, acc.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s));
-  HIPCK(st->ctx, hipMemcpyAsync(st->rho_s, rho.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
-  HIPCK(st->ctx, hipMemcpyAsync(st->prs_s, prs.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
-  HIPCK(st->ctx, hipMemcpyAsync(st->key_b, key.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
-  if (n != st->n) {
-    st->n = n;
-    st->graph_dirty = true;
-  }
-  st->host_dp.n = n;
-  HIPCK(st->ctx, hipMemcpyAsync(&st->dp->n, &st->host_dp.n, sizeof(uint32_t), hipMemcpyHostToDevice, s));
-  launch_aabb(st);
-  st->acc_zero = false;
-  st->fields_canonical = true;
-  st->stage = -1;
-  int rc = check_launch(st->ctx);
-  if (rc) return rc;
-  HIPCK(st->ctx, hipStreamSynchronize(s));
-  return CLSPH_OK;
+  return upload(st, 0, p, n);
 }
 
-static int download_common(clsph_state* st, clsph_particle* out, uint32_t n, bool sorted) {
-  if (!st || (!out && n)) return fail(st ? st->ctx : nullptr, CLSPH_EINVAL, "NULL argument");
-  if (n > st->n) return fail(st->ctx, CLSPH_EINVAL, "asked for more particles than are held");
-  if (n == 0) return device_errors(st);
-  if (n != st->n && !st->fields_canonical)
-    return fail(st->ctx, CLSPH_EINVAL, "partial downloads need the whole particle count");
-  std::vector<float4> pos(n), vel(n), vh(n), acc(n);
-  std::vector<float> rho(n), prs(n);
-  std::vector<uint32_t> key(n), slot(n);
-  hipStream_t s = st->ctx->stream;
-  const float4* P = sorted ? st->pos_s : st->pos;
-  const float4* V = sorted ? st->vel_s : st->vel;
-  const float4* H = sorted ? st->vh_s : st->vh;
-  HIPCK(st->ctx, hipMemcpyAsync(pos.data(), P, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
-  HIPCK(st->ctx, hipMemcpyAsync(vel.data(), V, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
-  HIPCK(st->ctx, hipMemcpyAsync(vh.data(), H, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
-  bool want_acc = sorted || !st->acc_zero;
-  if (want_acc)
-    HIPCK(st->ctx, hipMemcpyAsync(acc.data(), st->acc_s, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
-  HIPCK(st->ctx, hipMemcpyAsync(rho.data(), st->rho_s, sizeof(float) * n, hipMemcpyDeviceToHost, s));
-  HIPCK(st->ctx, hipMemcpyAsync(prs.data(), st->prs_s, sizeof(float) * n, hipMemcpyDeviceToHost, s));
-  HIPCK(st->ctx, hipMemcpyAsync(key.data(), st->key_b, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
-  bool reorder = !st->fields_canonical;
-  if (reorder)
-    HIPCK(st->ctx, hipMemcpyAsync(slot.data(), st->slot, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
-  HIPCK(st->ctx, hipStreamSynchronize(s));
-  // Fields computed in half-cell order go to each particle's canonical slot;
-  // the sorted-order arrays themselves are also in half-cell order.
-  for (uint32_t k = 0; k < n; ++k) {
-    uint32_t o = reorder ? slot[k] : k;
-    if (o >= n) return fail(st->ctx, CLSPH_EDEVICE, "canonical slots are not a permutation");
-    clsph_particle& q = out[o];
-    memset(&q, 0, sizeof(q));
-    q.density = rho[k];
-    q.pressure = prs[k];
-    q.grid_index = reorder ? key[k] >> 3 : key[k];
-    if (want_acc && (sorted || !reorder)) {
-      q.acceleration.s[0] = acc[k].x, q.acceleration.s[1] = acc[k].y, q.acceleration.s[2] = acc[k].z;
-    }
-  }
-  for (uint32_t k = 0; k < n; ++k) {
-    uint32_t o = (sorted && reorder) ? slot[k] : k;
-    clsph_particle& q = out[o];
-    q.position.s[0] = pos[k].x, q.position.s[1] = pos[k].y, q.position.s[2] = pos[k].z;
-    q.velocity.s[0] = vel[k].x, q.velocity.s[1] = vel[k].y, q.velocity.s[2] = vel[k].z;
-    q.intermediate_velocity.s[0] = vh[k].x, q.intermediate_velocity.s[1] = vh[k].y,
-    q.intermediate_velocity.s[2] = vh[k].z;
-  }
-  return device_errors(st);
+int clsph_batch_upload(clsph_state* st, uint32_t scene, const clsph_particle* p, uint32_t n) {
+  return upload(st, scene, p, n);
 }
 
 int clsph_download_aos(clsph_state* st, clsph_particle* p, uint32_t n) {
-  return download_common(st, p, n, false);
+  return download(st, 0, p, n, false);
+}
+
+int clsph_batch_download(clsph_state* st, uint32_t scene, clsph_particle* p, uint32_t n) {
+  return download(st, scene, p, n, false);
 }
 
 int clsph_download_sorted_aos(clsph_state* st, clsph_particle* p, uint32_t n) {
-  return download_common(st, p, n, true);
+  return download(st, 0, p, n, true);
 }
 
 int clsph_scene_upload(clsph_state* st, const float* face_normals, const float* vertices,
                        uint32_t vertex_count, const uint32_t* indices, uint32_t face_count) {
-  if (!st) return fail(nullptr, CLSPH_EINVAL, "NULL state");
-  if (face_count && (!face_normals || !vertices || !indices))
-    return fail(st->ctx, CLSPH_EINVAL, "NULL mesh array");
-  for (uint64_t k = 0; k < 3ull * face_count; ++k)
-    if (indices[k] >= vertex_count)
-      return fail(st->ctx, CLSPH_EINVAL, "triangle index out of range of the vertex array");
-  HIPCK(st->ctx, hipStreamSynchronize(st->ctx->stream));
-  if (st->face_normals) (void)hipFree(st->face_normals);
-  if (st->vertices) (void)hipFree(st->vertices);
-  if (st->indices) (void)hipFree(st->indices);
-  st->face_normals = nullptr;
-  st->vertices = nullptr;
-  st->indices = nullptr;
-  int rc = dalloc(st->ctx, &st->face_normals, 3ull * face_count);
-  if (!rc) rc = dalloc(st->ctx, &st->vertices, 3ull * vertex_count);
-  if (!rc) rc = dalloc(st->ctx, &st->indices, 3ull * face_count);
-  if (rc) return rc;
-  if (face_count) {
-    HIPCK(st->ctx, hipMemcpy(st->face_normals, face_normals, 12ull * face_count, hipMemcpyHostToDevice));
-    HIPCK(st->ctx, hipMemcpy(st->indices, indices, 12ull * face_count, hipMemcpyHostToDevice));
-  }
-  if (vertex_count)
-    HIPCK(st->ctx, hipMemcpy(st->vertices, vertices, 12ull * vertex_count, hipMemcpyHostToDevice));
-  st->face_count = face_count;
-  st->vert_floats = 3 * vertex_count;
-  st->graph_dirty = true;
-  return build_tri_grid(st, vertices, indices, face_count, face_count > 64 ? 0.f : -1.f);
+  return scene_upload(st, 0, face_normals, vertices, vertex_count, indices, face_count);
+}
+
+int clsph_batch_scene_upload(clsph_state* st, uint32_t scene, const float* face_normals,
+                             const float* vertices, uint32_t vertex_count,
+                             const uint32_t* indices, uint32_t face_count) {
+  return scene_upload(st, scene, face_normals, vertices, vertex_count, indices, face_count);
 }
 
 int clsph_scene_grid(clsph_state* st, float cell_size) {
   if (!st) return CLSPH_EINVAL;
-  HIPCK(st->ctx, hipStreamSynchronize(st->ctx->stream));
-  std::vector<float> verts(st->vert_floats);
-  std::vector<uint32_t> idx(3ull * st->face_count);
-  if (st->vert_floats)
-    HIPCK(st->ctx, hipMemcpy(verts.data(), st->vertices, 4ull * st->vert_floats, hipMemcpyDeviceToHost));
-  if (st->face_count)
-    HIPCK(st->ctx, hipMemcpy(idx.data(), st->indices, 12ull * st->face_count, hipMemcpyDeviceToHost));
-  st->graph_dirty = true;
-  return build_tri_grid(st, verts.data(), idx.data(), st->face_count, cell_size);
+  st->meshes[0].grid_cell = cell_size;
+  return rebuild_meshes(st);
 }
 
-int clsph_grid(clsph_state* st) { return st ? run_stage(st, 0) : CLSPH_EINVAL; }
-int clsph_hash(clsph_state* st) { return st ? run_stage(st, 1) : CLSPH_EINVAL; }
-int clsph_sort(clsph_state* st) { return st ? run_stage(st, 2) : CLSPH_EINVAL; }
-int clsph_cell_table(clsph_state* st) { return st ? run_stage(st, 3) : CLSPH_EINVAL; }
-int clsph_density_pressure(clsph_state* st) { return st ? run_stage(st, 4) : CLSPH_EINVAL; }
-int clsph_forces(clsph_state* st) { return st ? run_stage(st, 5) : CLSPH_EINVAL; }
-int clsph_advect_collide(clsph_state* st) { return st ? run_stage(st, 6) : CLSPH_EINVAL; }
-
-static int ensure_graph(clsph_state* st) {
-  if (!st->graph_dirty && st->graph) return CLSPH_OK;
-  if (st->graph) {
-    (void)hipGraphExecDestroy(st->graph);
-    st->graph = nullptr;
+int clsph_set_block_sizes(clsph_state* st, const uint32_t sizes[7]) {
+  if (!st || !sizes) return CLSPH_EINVAL;
+  int idx[kNumStages];
+  for (int s = 0; s < kNumStages; ++s) {
+    idx[s] = st->stage_bs[s];
+    if (sizes[s] == 0) continue;
+    int found = -1;
+    for (int b = 0; b < kNumBlockSizes; ++b)
+      if ((uint32_t)kBlockSizes[b] == sizes[s]) found = b;
+    if (found < 0) return fail(st->ctx, CLSPH_EINVAL, "block size must be 64, 128, 256 or 512");
+    idx[s] = found;
   }
-  hipStream_t s = st->ctx->stream;
-  hipGraph_t g = nullptr;
-  HIPCK(st->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  enqueue_step(st);
-  hipError_t le = hipGetLastError();
-  HIPCK(st->ctx, hipStreamEndCapture(s, &g));
-  if (le != hipSuccess) {
-    (void)hipGraphDestroy(g);
-    return fail(st->ctx, CLSPH_EDEVICE, std::string("capture: ") + hipGetErrorString(le));
+  HIPCK(st->ctx, hipStreamSynchronize(st->ctx->stream));
+  bool advect_changed = idx[kStageAdvect] != st->stage_bs[kStageAdvect];
+  for (int s = 0; s < kNumStages; ++s) st->stage_bs[s] = idx[s];
+  st->graph_dirty = true;
+  if (advect_changed) {
+    // the bounding-box partials are laid out per advection work-group:
+    // rebuild the scene ranges and recompute them from the current state
+    std::vector<DevParams> d;
+    int rc = read_dps(st, d);
+    if (!rc) rc = build_blocks(st);
+    if (rc) return rc;
+    for (uint32_t s = 0; s < st->n_scenes; ++s) {
+      d[s].block_begin = st->host_dps[s].block_begin;
+      d[s].block_end = st->host_dps[s].block_end;
+    }
+    HIPCK(st->ctx, hipMemcpy(st->dps, d.data(), sizeof(DevParams) * st->n_scenes,
+                             hipMemcpyHostToDevice));
+    launch_aabb(st);
+    rc = check_launch(st->ctx);
+    if (rc) return rc;
+    HIPCK(st->ctx, hipStreamSynchronize(st->ctx->stream));
   }
-  hipError_t ie = hipGraphInstantiate(&st->graph, g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
-  if (ie != hipSuccess) {
-    st->graph = nullptr;
-    return fail(st->ctx, CLSPH_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(ie));
-  }
-  st->graph_dirty = false;
   return CLSPH_OK;
+}
+
+int clsph_get_block_sizes(const clsph_state* st, uint32_t sizes[7]) {
+  if (!st || !sizes) return CLSPH_EINVAL;
+  for (int s = 0; s < kNumStages; ++s) sizes[s] = (uint32_t)st->bs_of(s);
+  return CLSPH_OK;
+}
+
+int clsph_grid(clsph_state* st) { return st ? run_stage(st, kStageGrid) : CLSPH_EINVAL; }
+int clsph_hash(clsph_state* st) { return st ? run_stage(st, kStageHash) : CLSPH_EINVAL; }
+int clsph_sort(clsph_state* st) { return st ? run_stage(st, kStageSort) : CLSPH_EINVAL; }
+int clsph_cell_table(clsph_state* st) { return st ? run_stage(st, kStageTable) : CLSPH_EINVAL; }
+int clsph_density_pressure(clsph_state* st) {
+  return st ? run_stage(st, kStageDensity) : CLSPH_EINVAL;
+}
+int clsph_forces(clsph_state* st) { return st ? run_stage(st, kStageForces) : CLSPH_EINVAL; }
+int clsph_advect_collide(clsph_state* st) {
+  return st ? run_stage(st, kStageAdvect) : CLSPH_EINVAL;
 }
 
 int clsph_step(clsph_state* st, uint32_t nsteps) {
@@ -549,68 +773,82 @@ int clsph_step(clsph_state* st, uint32_t nsteps) {
   for (uint32_t k = 0; k < nsteps; ++k) HIPCK(st->ctx, hipGraphLaunch(st->graph, st->ctx->stream));
   st->acc_zero = true;
   st->fields_canonical = false;
-  st->stage = 6;
+  st->stage = kStageAdvect;
+  return CLSPH_OK;
+}
+
+int clsph_batch_get_grid(clsph_state* st, uint32_t scene, clsph_simulation_parameters* params) {
+  if (!st || !params) return CLSPH_EINVAL;
+  if (scene >= st->n_scenes) return fail(st->ctx, CLSPH_EINVAL, "scene index out of range");
+  std::vector<DevParams> d;
+  int rc = read_dps(st, d);
+  if (rc) return rc;
+  *params = d[scene].p;
   return CLSPH_OK;
 }
 
 int clsph_get_grid(clsph_state* st, clsph_simulation_parameters* params) {
-  if (!st || !params) return CLSPH_EINVAL;
-  DevParams d;
-  HIPCK(st->ctx, hipMemcpyAsync(&d, st->dp, sizeof(d), hipMemcpyDeviceToHost, st->ctx->stream));
-  HIPCK(st->ctx, hipStreamSynchronize(st->ctx->stream));
-  *params = d.p;
-  return CLSPH_OK;
-}
-
-static int read_u32(clsph_state* st, const uint32_t* src, uint32_t* dst, size_t count) {
-  if (count == 0) return CLSPH_OK;
-  if (!dst) return fail(st->ctx, CLSPH_EINVAL, "NULL output");
-  HIPCK(st->ctx, hipMemcpyAsync(dst, src, sizeof(uint32_t) * count, hipMemcpyDeviceToHost,
-                                st->ctx->stream));
-  HIPCK(st->ctx, hipStreamSynchronize(st->ctx->stream));
-  return CLSPH_OK;
+  return clsph_batch_get_grid(st, 0, params);
 }
 
 int clsph_read_keys(clsph_state* st, uint32_t* keys, uint32_t n) {
   if (!st) return CLSPH_EINVAL;
-  if (n > st->n) return fail(st->ctx, CLSPH_EINVAL, "n exceeds the particle count");
-  int rc = read_u32(st, st->key_a, keys, n);
-  for (uint32_t i = 0; !rc && i < n; ++i) keys[i] >>= 3;  // half-cell key -> cell id
+  if (n > st->scene_n[0]) return fail(st->ctx, CLSPH_EINVAL, "n exceeds the particle count");
+  std::vector<DevParams> d;
+  int rc = read_dps(st, d);
+  if (!rc) rc = read_u32(st, st->key_a + st->scene_begin[0], keys, n);
+  for (uint32_t i = 0; !rc && i < n; ++i) keys[i] = (keys[i] - d[0].key_base) >> 3;
   return rc;
 }
 
 int clsph_read_sorted(clsph_state* st, uint32_t* keys, uint32_t* perm, uint32_t n) {
   if (!st) return CLSPH_EINVAL;
-  if (n != st->n) return fail(st->ctx, CLSPH_EINVAL, "n must be the particle count");
+  if (n != st->scene_n[0]) return fail(st->ctx, CLSPH_EINVAL, "n must be the particle count");
   std::vector<uint32_t> k(n), v(n), slot(n);
-  int rc = read_u32(st, st->key_b, k.data(), n);
-  if (!rc) rc = read_u32(st, st->val_b, v.data(), n);
-  if (!rc) rc = read_u32(st, st->slot, slot.data(), n);
+  std::vector<DevParams> d;
+  int rc = read_dps(st, d);
+  uint32_t b0 = st->scene_begin[0];
+  if (!rc) rc = read_u32(st, st->key_b + b0, k.data(), n);
+  if (!rc) rc = read_u32(st, st->val_b + b0, v.data(), n);
+  if (!rc) rc = read_u32(st, st->slot + b0, slot.data(), n);
   if (rc) return rc;
   // Half-cell order -> the reference's order (stable by cell id).
   for (uint32_t i = 0; i < n; ++i) {
-    uint32_t o = slot[i];
+    uint32_t o = slot[i] - b0;
     if (o >= n) return fail(st->ctx, CLSPH_EDEVICE, "canonical slots are not a permutation");
-    if (keys) keys[o] = k[i] >> 3;
-    if (perm) perm[o] = v[i];
+    if (keys) keys[o] = (k[i] - d[0].key_base) >> 3;
+    if (perm) perm[o] = v[i] - b0;
   }
   return CLSPH_OK;
 }
 
 int clsph_read_cell_table(clsph_state* st, uint32_t* table, uint32_t count) {
   if (!st) return CLSPH_EINVAL;
-  if (count > st->table_cap) return fail(st->ctx, CLSPH_EINVAL, "count exceeds table capacity");
   if (count == 0) return device_errors(st);
-  std::vector<uint32_t> ht(8ull * count);
-  int rc = read_u32(st, st->table, ht.data(), ht.size());
+  if (!table) return fail(st->ctx, CLSPH_EINVAL, "NULL output");
+  std::vector<DevParams> d;
+  int rc = read_dps(st, d);
   if (rc) return rc;
-  for (uint32_t c = 0; c < count; ++c) table[c] = ht[8ull * c];  // T[c] = ht[8c]
+  if ((uint64_t)d[0].key_base + 8ull * count > st->table_cap)
+    return fail(st->ctx, CLSPH_EINVAL, "count exceeds the table");
+  std::vector<uint32_t> ht(8ull * count);
+  rc = read_u32(st, st->table + d[0].key_base, ht.data(), ht.size());
+  if (rc) return rc;
+  for (uint32_t c = 0; c < count; ++c) table[c] = ht[8ull * c] - st->scene_begin[0];
   return device_errors(st);
 }
 
-int clsph_bounce_capped(clsph_state* st, uint32_t* count) {
+int clsph_batch_bounce_capped(clsph_state* st, uint32_t scene, uint32_t* count) {
   if (!st || !count) return CLSPH_EINVAL;
-  return read_u32(st, &st->dp->capped, count, 1);
+  if (scene >= st->n_scenes) return fail(st->ctx, CLSPH_EINVAL, "scene index out of range");
+  std::vector<DevParams> d;
+  int rc = read_dps(st, d);
+  if (!rc) *count = d[scene].capped;
+  return rc;
+}
+
+int clsph_bounce_capped(clsph_state* st, uint32_t* count) {
+  return clsph_batch_bounce_capped(st, 0, count);
 }
 
 int clsph_time_steps(clsph_state* st, uint32_t nsteps, float* ms) {
@@ -629,7 +867,7 @@ int clsph_time_steps(clsph_state* st, uint32_t nsteps, float* ms) {
   (void)hipEventDestroy(b);
   st->acc_zero = true;
   st->fields_canonical = false;
-  st->stage = 6;
+  st->stage = kStageAdvect;
   return device_errors(st);
 }
 
@@ -655,29 +893,28 @@ int clsph_time_stage(clsph_state* st, int stage, uint32_t reps, float* ms) {
 
 int clsph_profile_steps(clsph_state* st, uint32_t nsteps, float* stage_ms) {
   if (!st || !stage_ms) return CLSPH_EINVAL;
-  constexpr int kStages = 7;
-  std::vector<hipEvent_t> ev((size_t)nsteps * (kStages + 1));
+  std::vector<hipEvent_t> ev((size_t)nsteps * (kNumStages + 1));
   for (auto& e : ev) HIPCK(st->ctx, hipEventCreate(&e));
   for (uint32_t k = 0; k < nsteps; ++k) {
-    hipEvent_t* e = ev.data() + (size_t)k * (kStages + 1);
+    hipEvent_t* e = ev.data() + (size_t)k * (kNumStages + 1);
     HIPCK(st->ctx, hipEventRecord(e[0], st->ctx->stream));
-    for (int s = 0; s < kStages; ++s) {
+    for (int s = 0; s < kNumStages; ++s) {
       int rc = run_stage(st, s);
       if (rc) return rc;
       HIPCK(st->ctx, hipEventRecord(e[s + 1], st->ctx->stream));
     }
   }
-  HIPCK(st->ctx, hipEventSynchronize(ev.back()));
-  for (int s = 0; s < kStages; ++s) stage_ms[s] = 0.f;
+  if (!ev.empty()) HIPCK(st->ctx, hipEventSynchronize(ev.back()));
+  for (int s = 0; s < kNumStages; ++s) stage_ms[s] = 0.f;
   for (uint32_t k = 0; k < nsteps; ++k) {
-    hipEvent_t* e = ev.data() + (size_t)k * (kStages + 1);
-    for (int s = 0; s < kStages; ++s) {
+    hipEvent_t* e = ev.data() + (size_t)k * (kNumStages + 1);
+    for (int s = 0; s < kNumStages; ++s) {
       float t = 0.f;
       HIPCK(st->ctx, hipEventElapsedTime(&t, e[s], e[s + 1]));
       stage_ms[s] += t;
     }
   }
-  for (int s = 0; s < kStages; ++s) stage_ms[s] /= (float)(nsteps ? nsteps : 1);
+  for (int s = 0; s < kNumStages; ++s) stage_ms[s] /= (float)(nsteps ? nsteps : 1);
   for (auto& e : ev) (void)hipEventDestroy(e);
   return device_errors(st);
 }

@@ -8,46 +8,72 @@
 
 namespace clsph {
 
-// Error bits raised by kernels into DevParams::err; read back by the host at
-// synchronisation points.
+// Error bits raised by kernels into DevParams::err / BatchParams::err; read
+// back by the host at synchronisation points.
 enum : uint32_t {
   kErrGridDim = 1u,        // a grid dimension reached 1024 (sph_simulation.cpp:247-249)
-  kErrTableCap = 2u,       // grid_cell_count exceeds the cell table allocation
-  kErrFineKey = 4u,        // half-cell keys need more than 32 bits (grid_cell_count >= 2^29)
+  kErrTableCap = 2u,       // the cells of all scenes exceed the table allocation
+  kErrFineKey = 4u,        // half-cell keys need more than 32 bits
 };
 
 // Particles are sorted by the Morton key of the half-size cell (side h) they
 // fall in. Its top bits are the reference's cell id (side 2h), so the order
 // groups each cell's particles by octant; the reference's stable order by
-// cell id is recovered from it per particle.
+// cell id is recovered from it per particle. The scenes of a batch occupy
+// consecutive key ranges, so one sort orders every scene.
 constexpr int kSortPasses = 3;       // LSD passes; digit width adapts to the key width
 constexpr int kMaxDigitBits = 11;    // 3 x 11 bits covers any 32-bit key
 constexpr int kMaxBins = 1 << kMaxDigitBits;
 // Neighbour-list slots per particle; a particle with more neighbours is
 // handled by forces with a cell walk instead.
 constexpr uint32_t kMaxNeighbours = 128;
+// Work-group sizes a per-particle kernel can be launched with.
+constexpr int kNumBlockSizes = 4;
+constexpr int kBlockSizes[kNumBlockSizes] = {64, 128, 256, 512};
+constexpr int kMaxBlock = 512;
 
-// Device-resident parameter block. `p` mirrors the reference's
-// simulation_parameters; its grid fields (min/max point, grid sizes,
-// grid_cell_count) are written on the device every step.
-struct DevParams {
-  clsph_simulation_parameters p;   // 128 B
-  clsph_precomputed_kernel_values t;
-  uint32_t n;            // particle count
-  uint32_t table_cap;    // cell capacity of the tables (half-cell table holds 8x + 1)
-  uint32_t key_bits;     // width of the half-cell keys: bitlen(grid_cell_count) + 3
-  uint32_t digit_bits;   // per LSD pass
-  uint32_t err;          // kErr* bits, sticky until cleared by the host
-  uint32_t capped;       // particles whose collision loop hit max_bounces (last step)
-  uint32_t max_bounces;
-};
-
-// Uniform grid over the collision mesh (cells x fastest, then y, then z).
+// Uniform grid over a scene's collision mesh (cells x fastest, then y, then
+// z); cell_base offsets into the batch's concatenated cell lists.
 struct TriGrid {
   float origin[3];
   float inv_cell;
   uint32_t dims[3];
   uint32_t enabled;
+  uint32_t cell_base;
+};
+
+// Device-resident parameters of one scene. `p` mirrors the reference's
+// simulation_parameters; its grid fields (min/max point, grid sizes,
+// grid_cell_count) are written on the device every substep.
+struct DevParams {
+  clsph_simulation_parameters p;   // 128 B
+  clsph_precomputed_kernel_values t;
+  uint32_t begin, n;     // the scene's particles: [begin, begin + n) in every order
+  uint32_t key_base;     // first half-cell key of the scene (a multiple of 8)
+  uint32_t key_end;      // key_base + 8 * grid_cell_count
+  uint32_t err;          // kErr* bits of this scene, sticky until cleared by the host
+  uint32_t capped;       // particles whose collision loop hit max_bounces (last substep)
+  uint32_t max_bounces;
+  uint32_t face_begin, face_count;  // the scene's triangles in the batch mesh
+  uint32_t block_begin, block_end;  // the scene's work-groups in the bounding-box table
+  TriGrid tgrid;
+};
+
+// Batch-wide values, computed on the device each substep.
+struct BatchParams {
+  uint32_t n_scenes;
+  uint32_t n;            // particles in all scenes
+  uint32_t key_total;    // half-cell keys of all scenes: 8 * sum(grid_cell_count + 1)
+  uint32_t table_cap;    // half-cell table capacity in keys
+  uint32_t key_bits;     // bitlen(key_total)
+  uint32_t digit_bits;   // per LSD pass
+  uint32_t err;
+};
+
+// Work-group descriptor for per-particle kernels: scene and first particle.
+struct BlockDesc {
+  uint32_t scene;
+  uint32_t first;
 };
 
 __host__ __device__ inline uint32_t spread10(uint32_t v) {
@@ -69,6 +95,17 @@ __host__ __device__ inline uint32_t bitlen(uint32_t v) {
     v >>= 1;
   }
   return b;
+}
+
+// The particle a thread of a per-particle kernel handles, or UINT32_MAX when
+// the thread is past the end of its scene.
+__device__ inline uint32_t particle_of(const BlockDesc* __restrict__ blocks,
+                                       const DevParams* __restrict__ dps, uint32_t& scene) {
+  BlockDesc b = blocks[blockIdx.x];
+  scene = b.scene;
+  uint32_t i = b.first + threadIdx.x;
+  const DevParams& d = dps[b.scene];
+  return i < d.begin + d.n ? i : 0xFFFFFFFFu;
 }
 
 }  // namespace clsph

@@ -1,5 +1,6 @@
 // Grid stages: bounding box, grid parameters, cell-id hashing, state
-// permutation, cell tables and the reference's stable order.
+// permutation, cell tables and the reference's stable order, for every scene
+// of a batch at once.
 //
 // The bounding box / grid arithmetic must reproduce the reference's host FP32
 // sequence bit for bit (sph_simulation.cpp:201-252), and the cell id the
@@ -10,67 +11,35 @@
 // (uint)u and the half-cell coordinate is (uint)(2u) = 2 (uint)u + [frac(u) >=
 // 1/2], exact because 2u is exact. The Morton key of the half-cell is then
 // (reference key << 3) | octant, so sorting by it keeps the reference's cells
-// contiguous and in order, each grouped by octant.
+// contiguous and in order, each grouped by octant. A scene's keys are offset
+// by its key_base.
+#include "reduce.h"
 #include "state.h"
+
+#pragma clang fp contract(off)
 
 namespace clsph {
 
-constexpr int kBlock = 256;
-
-__device__ inline float wave_min(float v) {
-  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ inline float wave_max(float v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
-// Block-level min/max of (x, y, z) written to partial[blockIdx.x * 6 + k].
-// NaN coordinates are skipped, like the reference's `<` / `>` tests.
-__device__ inline void block_aabb_partial(float4 p, bool valid, float* partial) {
-  __shared__ float red[6][kBlock / 64];
-  const float inf = __builtin_huge_valf();
-  float v[6];
-  v[0] = (valid && p.x == p.x) ? p.x : inf;
-  v[1] = (valid && p.y == p.y) ? p.y : inf;
-  v[2] = (valid && p.z == p.z) ? p.z : inf;
-  v[3] = (valid && p.x == p.x) ? p.x : -inf;
-  v[4] = (valid && p.y == p.y) ? p.y : -inf;
-  v[5] = (valid && p.z == p.z) ? p.z : -inf;
-  for (int k = 0; k < 3; ++k) v[k] = wave_min(v[k]);
-  for (int k = 3; k < 6; ++k) v[k] = wave_max(v[k]);
-  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  if (l == 0)
-    for (int k = 0; k < 6; ++k) red[k][w] = v[k];
-  __syncthreads();
-  if (threadIdx.x < 6) {
-    int k = threadIdx.x;
-    float r = red[k][0];
-    for (int q = 1; q < kBlock / 64; ++q)
-      r = (k < 3) ? fminf(r, red[k][q]) : fmaxf(r, red[k][q]);
-    partial[blockIdx.x * 6 + k] = r;
-  }
+__global__ __launch_bounds__(kMaxBlock) void k_aabb(const float4* __restrict__ pos,
+                                                    const BlockDesc* __restrict__ blocks,
+                                                    const DevParams* __restrict__ dps,
+                                                    float* __restrict__ partial) {
+  uint32_t scene;
+  uint32_t i = particle_of(blocks, dps, scene);
+  bool valid = i != 0xFFFFFFFFu;
+  float4 p = valid ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  block_aabb_partial(p, valid, partial);
 }
 
-__global__ __launch_bounds__(kBlock) void k_aabb(const float4* __restrict__ pos,
-                                                 uint32_t n,
-                                                 float* __restrict__ partial) {
-  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  float4 p = (i < n) ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-  block_aabb_partial(p, i < n, partial);
-}
-
-// One block: reduces the per-block partial boxes, then restates
+// One work-group per scene: reduces the scene's partial boxes, then restates
 // sph_simulation.cpp:201-252 in FP32.
-__global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dp,
-                                                      const float* __restrict__ partial,
-                                                      uint32_t n_partial) {
-#pragma clang fp contract(off)
+__global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dps,
+                                                      const float* __restrict__ partial) {
   __shared__ float red[6][16];
+  DevParams* dp = dps + blockIdx.x;
   const float inf = __builtin_huge_valf();
   float v[6] = {inf, inf, inf, -inf, -inf, -inf};
-  for (uint32_t b = threadIdx.x; b < n_partial; b += blockDim.x) {
+  for (uint32_t b = dp->block_begin + threadIdx.x; b < dp->block_end; b += blockDim.x) {
     for (int k = 0; k < 3; ++k) v[k] = fminf(v[k], partial[b * 6 + k]);
     for (int k = 3; k < 6; ++k) v[k] = fmaxf(v[k], partial[b * 6 + k]);
   }
@@ -86,6 +55,11 @@ __global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dp
     float r = red[k][0];
     for (int q = 1; q < nw; ++q) r = (k < 3) ? fminf(r, red[k][q]) : fmaxf(r, red[k][q]);
     v[k] = r;
+  }
+  dp->capped = 0;
+  if (dp->n == 0) {
+    dp->p.grid_cell_count = 0;
+    return;
   }
   // Initial values are numeric_limits<cl_int>::max()/min() as float; a
   // position replaces them only when strictly smaller / larger.
@@ -111,26 +85,48 @@ __global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dp
   dp->p.grid_size_z = (int32_t)gs[2];
   uint32_t G = morton3(gs[0], gs[1], gs[2]);
   dp->p.grid_cell_count = G;
-  if (G > dp->table_cap) err |= kErrTableCap;
-  uint32_t kb = bitlen(G) + 3;
-  if (kb > 32) err |= kErrFineKey;
-  dp->key_bits = kb;
-  uint32_t db = (kb + kSortPasses - 1) / kSortPasses;
-  dp->digit_bits = db > (uint32_t)kMaxDigitBits ? (uint32_t)kMaxDigitBits : db;
-  dp->capped = 0;
+  if (G >= (1u << 29)) err |= kErrFineKey;
   if (err) atomicOr(&dp->err, err);
 }
 
+// One thread: lays the scenes' half-cell key ranges end to end (each scene
+// gets 8 (G + 1) keys) and sizes the sort digits for the total.
+__global__ void k_batch_keys(DevParams* __restrict__ dps, BatchParams* __restrict__ bp) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint64_t base = 0;
+  uint32_t err = 0;
+  for (uint32_t s = 0; s < bp->n_scenes; ++s) {
+    DevParams* dp = dps + s;
+    uint64_t G = (dp->err & kErrFineKey) ? 0u : dp->p.grid_cell_count;
+    dp->key_base = (uint32_t)base;
+    dp->key_end = (uint32_t)(base + 8ull * G);
+    base += 8ull * (G + 1);
+    if (base > 0xFFFFFFF0ull) {
+      err |= kErrFineKey;
+      base = 0xFFFFFFF0ull;
+    }
+  }
+  if (base > bp->table_cap) err |= kErrTableCap;
+  bp->key_total = (uint32_t)base;
+  uint32_t kb = bitlen((uint32_t)base);
+  bp->key_bits = kb;
+  uint32_t db = (kb + kSortPasses - 1) / kSortPasses;
+  bp->digit_bits = db < 1 ? 1 : (db > (uint32_t)kMaxDigitBits ? (uint32_t)kMaxDigitBits : db);
+  if (err) atomicOr(&bp->err, err);
+}
+
 // locate_in_grid (grid.cl:43-67): the reference's cell id is
-// Morton((uint)((x - min) / (h * 2))); the half-cell key adds the octant.
-__global__ __launch_bounds__(kBlock) void k_hash(const float4* __restrict__ pos,
-                                                 const DevParams* __restrict__ dp,
-                                                 uint32_t* __restrict__ key,
-                                                 uint32_t* __restrict__ val,
-                                                 uint32_t n) {
-#pragma clang fp contract(off)
-  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+// Morton((uint)((x - min) / (h * 2))); the half-cell key adds the octant and
+// the scene's key base.
+__global__ __launch_bounds__(kMaxBlock) void k_hash(const float4* __restrict__ pos,
+                                                    const BlockDesc* __restrict__ blocks,
+                                                    const DevParams* __restrict__ dps,
+                                                    uint32_t* __restrict__ key,
+                                                    uint32_t* __restrict__ val) {
+  uint32_t scene;
+  uint32_t i = particle_of(blocks, dps, scene);
+  if (i == 0xFFFFFFFFu) return;
+  const DevParams* dp = dps + scene;
   float4 p = pos[i];
   float L = dp->p.h * 2;
   float u[3] = {(p.x - dp->p.min_point.s[0]) / L, (p.y - dp->p.min_point.s[1]) / L,
@@ -140,17 +136,17 @@ __global__ __launch_bounds__(kBlock) void k_hash(const float4* __restrict__ pos,
     c[a] = (uint32_t)u[a];
     oct |= (((uint32_t)(2.f * u[a])) & 1u) << a;
   }
-  key[i] = (morton3(c[0], c[1], c[2]) << 3) | oct;
+  key[i] = dp->key_base + ((morton3(c[0], c[1], c[2]) << 3) | oct);
   val[i] = i;
 }
 
 // Gathers the canonical state into sorted order: s[i] = canonical[perm[i]].
-__global__ __launch_bounds__(kBlock) void k_permute(
+__global__ __launch_bounds__(256) void k_permute(
     const uint32_t* __restrict__ perm, const float4* __restrict__ pos,
     const float4* __restrict__ vel, const float4* __restrict__ vh,
     float4* __restrict__ pos_s, float4* __restrict__ vel_s,
     float4* __restrict__ vh_s, uint32_t n) {
-  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   uint32_t j = perm[i];
   pos_s[i] = pos[j];
@@ -171,19 +167,22 @@ __device__ inline uint32_t lower_bound(const uint32_t* __restrict__ keys, uint32
   return lo;
 }
 
-// Half-cell table: ht[f] = #{keys < f} for f in [0, 8G], one thread per cell
-// of side 2h. The reference's cell table (sph_simulation.cpp:162-170),
-// T[c] = #{cell ids < c}, is ht[8c].
-__global__ __launch_bounds__(kBlock) void k_cell_table(const uint32_t* __restrict__ keys,
-                                                       uint32_t n,
-                                                       const DevParams* __restrict__ dp,
-                                                       uint32_t* __restrict__ ht) {
-  uint32_t G = dp->p.grid_cell_count;
-  if (G > dp->table_cap || (dp->err & kErrFineKey)) return;
-  for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < G; c += gridDim.x * kBlock) {
-    uint32_t start = lower_bound(keys, 0, n, c << 3);
+// Half-cell table: ht[f] = #{keys < f} for f in [0, key_total], one thread
+// per group of 8 keys (a cell of side 2h). The reference's cell table of a
+// scene (sph_simulation.cpp:162-170), T[c] = #{cell ids < c}, is
+// ht[key_base + 8c] - begin.
+__global__ __launch_bounds__(256) void k_cell_table(const uint32_t* __restrict__ keys,
+                                                    uint32_t n,
+                                                    const BatchParams* __restrict__ bp,
+                                                    uint32_t* __restrict__ ht) {
+  uint32_t total = bp->key_total;
+  if (total > bp->table_cap || (bp->err & kErrFineKey)) return;
+  uint32_t cells = total >> 3;
+  for (uint32_t c = blockIdx.x * 256 + threadIdx.x; c < cells; c += gridDim.x * 256) {
+    uint32_t k0 = c << 3;
+    uint32_t start = lower_bound(keys, 0, n, k0);
     // end of the cell: gallop from start, then bisect
-    uint32_t nxt = (c + 1) << 3;
+    uint32_t nxt = k0 + 8u;
     uint32_t lo = start, step = 64, hi = start;
     while (hi < n && keys[hi] < nxt) {
       lo = hi + 1;
@@ -191,13 +190,13 @@ __global__ __launch_bounds__(kBlock) void k_cell_table(const uint32_t* __restric
       step <<= 1;
     }
     uint32_t end = lower_bound(keys, lo, hi, nxt);
-    ht[8u * c] = start;
+    ht[k0] = start;
     uint32_t b = start;
     for (uint32_t o = 1; o < 8; ++o) {
-      b = lower_bound(keys, b, end, (c << 3) | o);
-      ht[8u * c + o] = b;
+      b = lower_bound(keys, b, end, k0 | o);
+      ht[k0 + o] = b;
     }
-    if (c == G - 1) ht[8u * G] = n;
+    if (c == cells - 1) ht[total] = n;
   }
 }
 
@@ -206,25 +205,23 @@ __global__ __launch_bounds__(kBlock) void k_cell_table(const uint32_t* __restric
 // (input) order kept inside each octant, so the particle's rank in its cell
 // is its index in its octant plus the members of the other octants that came
 // before it in the input.
-__global__ __launch_bounds__(kBlock) void k_canonical_slot(const uint32_t* __restrict__ keys,
-                                                           const uint32_t* __restrict__ perm,
-                                                           const uint32_t* __restrict__ ht,
-                                                           const DevParams* __restrict__ dp,
-                                                           uint32_t* __restrict__ slot,
-                                                           uint32_t n) {
-  uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(256) void k_canonical_slot(const uint32_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ perm,
+                                                        const uint32_t* __restrict__ ht,
+                                                        const BatchParams* __restrict__ bp,
+                                                        uint32_t* __restrict__ slot, uint32_t n) {
+  uint32_t k = blockIdx.x * 256 + threadIdx.x;
   if (k >= n) return;
-  uint32_t G = dp->p.grid_cell_count;
-  if (G > dp->table_cap || (dp->err & kErrFineKey)) {
+  if (bp->key_total > bp->table_cap || (bp->err & kErrFineKey)) {
     slot[k] = k;
     return;
   }
-  uint32_t f = keys[k], c = f >> 3, o = f & 7u;
+  uint32_t f = keys[k], c8 = f & ~7u, o = f & 7u;
   uint32_t me = perm[k];
   uint32_t rank = k - ht[f];
   for (uint32_t q = 0; q < 8; ++q) {
     if (q == o) continue;
-    uint32_t lo = ht[8u * c + q], hi = ht[8u * c + q + 1];
+    uint32_t lo = ht[c8 + q], hi = ht[c8 + q + 1];
     uint32_t a = lo, b = hi;
     while (a < b) {
       uint32_t mid = (a + b) >> 1;
@@ -235,41 +232,41 @@ __global__ __launch_bounds__(kBlock) void k_canonical_slot(const uint32_t* __res
     }
     rank += a - lo;
   }
-  slot[k] = ht[8u * c] + rank;
+  slot[k] = ht[c8] + rank;
 }
-
-static inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 void launch_aabb(clsph_state* s) {
   if (s->n == 0) return;
-  hipLaunchKernelGGL(k_aabb, dim3(blocks_for(s->n)), dim3(kBlock), 0, s->ctx->stream,
-                     s->pos, s->n, s->aabb_partial);
+  hipLaunchKernelGGL(k_aabb, dim3(s->nblocks_of(kStageAdvect)), dim3(s->bs_of(kStageAdvect)), 0,
+                     s->ctx->stream, s->pos, s->table_of(kStageAdvect), s->dps, s->aabb_partial);
 }
 
 void launch_grid_params(clsph_state* s) {
-  hipLaunchKernelGGL(k_grid_params, dim3(1), dim3(1024), 0, s->ctx->stream, s->dp,
-                     s->aabb_partial, blocks_for(s->n));
+  hipLaunchKernelGGL(k_grid_params, dim3(s->n_scenes), dim3(1024), 0, s->ctx->stream, s->dps,
+                     s->aabb_partial);
+  hipLaunchKernelGGL(k_batch_keys, dim3(1), dim3(64), 0, s->ctx->stream, s->dps, s->bp);
 }
 
 void launch_hash(clsph_state* s) {
   if (s->n == 0) return;
-  hipLaunchKernelGGL(k_hash, dim3(blocks_for(s->n)), dim3(kBlock), 0, s->ctx->stream,
-                     s->pos, s->dp, s->key_a, s->val_a, s->n);
+  hipLaunchKernelGGL(k_hash, dim3(s->nblocks_of(kStageHash)), dim3(s->bs_of(kStageHash)), 0,
+                     s->ctx->stream, s->pos, s->table_of(kStageHash), s->dps, s->key_a,
+                     s->val_a);
 }
 
 void launch_permute(clsph_state* s) {
   if (s->n == 0) return;
-  hipLaunchKernelGGL(k_permute, dim3(blocks_for(s->n)), dim3(kBlock), 0, s->ctx->stream,
+  hipLaunchKernelGGL(k_permute, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream,
                      s->val_b, s->pos, s->vel, s->vh, s->pos_s, s->vel_s, s->vh_s, s->n);
 }
 
 void launch_cell_table(clsph_state* s) {
   if (s->n == 0) return;
   uint32_t blocks = (uint32_t)s->ctx->cus * 8;
-  hipLaunchKernelGGL(k_cell_table, dim3(blocks), dim3(kBlock), 0, s->ctx->stream,
-                     s->key_b, s->n, s->dp, s->table);
-  hipLaunchKernelGGL(k_canonical_slot, dim3(blocks_for(s->n)), dim3(kBlock), 0,
-                     s->ctx->stream, s->key_b, s->val_b, s->table, s->dp, s->slot, s->n);
+  hipLaunchKernelGGL(k_cell_table, dim3(blocks), dim3(256), 0, s->ctx->stream, s->key_b, s->n,
+                     s->bp, s->table);
+  hipLaunchKernelGGL(k_canonical_slot, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream,
+                     s->key_b, s->val_b, s->table, s->bp, s->slot, s->n);
 }
 
 }  // namespace clsph

@@ -26,9 +26,9 @@ __device__ inline uint32_t digit_of(uint32_t key, uint32_t shift, uint32_t mask)
 
 // Digit width of this sort, clamped to max_db (the width the launch sized its
 // shared memory for; a wider one only arises with the table-overflow error set).
-__device__ inline void pass_digit(const DevParams* __restrict__ dp, uint32_t max_db, int pass,
+__device__ inline void pass_digit(const BatchParams* __restrict__ bp, uint32_t max_db, int pass,
                                   uint32_t& nb, uint32_t& mask, uint32_t& shift) {
-  uint32_t db = dp->digit_bits;
+  uint32_t db = bp->digit_bits;
   db = db > max_db ? max_db : db;
   nb = 1u << db;
   mask = nb - 1;
@@ -38,13 +38,13 @@ __device__ inline void pass_digit(const DevParams* __restrict__ dp, uint32_t max
 // Per-tile digit histogram, stored digit-major: hist[d * n_tiles + tile].
 __global__ __launch_bounds__(kSortBlock) void k_sort_count(const uint32_t* __restrict__ keys,
                                                            uint32_t n,
-                                                           const DevParams* __restrict__ dp,
+                                                           const BatchParams* __restrict__ bp,
                                                            uint32_t max_db, int pass,
                                                            uint32_t* __restrict__ hist,
                                                            uint32_t n_tiles) {
   extern __shared__ uint32_t bins[];  // 1 << max_db entries
   uint32_t nb, mask, shift;
-  pass_digit(dp, max_db, pass, nb, mask, shift);
+  pass_digit(bp, max_db, pass, nb, mask, shift);
   for (uint32_t d = threadIdx.x; d < nb; d += kSortBlock) bins[d] = 0;
   __syncthreads();
   uint32_t base = blockIdx.x * kTile;
@@ -80,12 +80,12 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v) {
 // place, and the digit's total.
 __global__ __launch_bounds__(kSortBlock) void k_sort_scan(uint32_t* __restrict__ hist,
                                                           uint32_t n_tiles,
-                                                          const DevParams* __restrict__ dp,
+                                                          const BatchParams* __restrict__ bp,
                                                           uint32_t max_db,
                                                           uint32_t* __restrict__ totals) {
   uint32_t d = blockIdx.x;
   uint32_t nb, mask, shift;
-  pass_digit(dp, max_db, 0, nb, mask, shift);
+  pass_digit(bp, max_db, 0, nb, mask, shift);
   if (d >= nb) return;
   uint32_t* row = hist + (size_t)d * n_tiles;
   uint32_t per = (n_tiles + kSortBlock - 1) / kSortBlock;
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scan(uint32_t* __restrict__
 __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n,
-    const DevParams* __restrict__ dp, uint32_t max_db, int pass,
+    const BatchParams* __restrict__ bp, uint32_t max_db, int pass,
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ totals,
     uint32_t n_tiles) {
   // Shared memory: kWaves running counts per digit, then the digit starts.
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(
   uint32_t* wcnt = smem;                         // wcnt[q * nbmax + d]
   uint32_t* digit_start = smem + kWaves * nbmax;
   uint32_t nb, mask, shift;
-  pass_digit(dp, max_db, pass, nb, mask, shift);
+  pass_digit(bp, max_db, pass, nb, mask, shift);
   uint32_t db = 31u - (uint32_t)__builtin_clz(nb);
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (uint32_t d = threadIdx.x; d < nb; d += kSortBlock)
@@ -202,9 +202,9 @@ void launch_sort(clsph_state* s) {
   uint32_t* vin = s->val_a;
   uint32_t* kout = s->key_b;
   uint32_t* vout = s->val_b;
-  // Widest digit the state can need: keys stay below 8 * table_cap while the
-  // cell count fits the tables.
-  uint32_t kb = bitlen(s->table_cap) + 3;
+  // Widest digit the state can need: keys stay below table_cap while the
+  // cells fit the table.
+  uint32_t kb = bitlen(s->table_cap);
   uint32_t max_db = (kb + kSortPasses - 1) / kSortPasses;
   max_db = max_db > (uint32_t)kMaxDigitBits ? (uint32_t)kMaxDigitBits : max_db;
   uint32_t nbmax = 1u << max_db;
@@ -212,11 +212,11 @@ void launch_sort(clsph_state* s) {
   size_t scatter_lds = sizeof(uint32_t) * nbmax * (kWaves + 1);
   for (int pass = 0; pass < kSortPasses; ++pass) {
     hipLaunchKernelGGL(k_sort_count, dim3(nt), dim3(kSortBlock), count_lds, st, kin, s->n,
-                       s->dp, max_db, pass, s->hist, nt);
-    hipLaunchKernelGGL(k_sort_scan, dim3(nbmax), dim3(kSortBlock), 0, st, s->hist, nt, s->dp,
+                       s->bp, max_db, pass, s->hist, nt);
+    hipLaunchKernelGGL(k_sort_scan, dim3(nbmax), dim3(kSortBlock), 0, st, s->hist, nt, s->bp,
                        max_db, s->digit_base);
     hipLaunchKernelGGL(k_sort_scatter, dim3(nt), dim3(kSortBlock), scatter_lds, st, kin, vin,
-                       kout, vout, s->n, s->dp, max_db, pass, s->hist, s->digit_base, nt);
+                       kout, vout, s->n, s->bp, max_db, pass, s->hist, s->digit_base, nt);
     uint32_t* tk = kin;
     kin = kout;
     kout = tk;

@@ -12,8 +12,6 @@
 
 namespace clsph {
 
-constexpr int kBlock = 256;
-
 // Half-cell coordinates, with the hash's arithmetic (grid.cl:55-61):
 // (uint)(2 (x - min) / (2h)).
 __device__ inline void half_cell(float4 x, const DevParams* __restrict__ dp, uint32_t hc[3]) {
@@ -24,16 +22,16 @@ __device__ inline void half_cell(float4 x, const DevParams* __restrict__ dp, uin
   hc[2] = (uint32_t)(2.f * ((x.z - dp->p.min_point.s[2]) / L));
 }
 
-// Morton key of a half-cell: (cell id << 3) | octant.
+// Morton key of a half-cell within its scene: (cell id << 3) | octant.
 __device__ inline uint32_t half_key(uint32_t hx, uint32_t hy, uint32_t hz) {
   return (morton3(hx >> 1, hy >> 1, hz >> 1) << 3) | ((hz & 1u) << 2) | ((hy & 1u) << 1) |
          (hx & 1u);
 }
 
-// Particles of the half-cells [f, f + len) (Morton-adjacent), from the
-// half-cell table ht (F = 8 * grid_cell_count entries, ht[F] = n). Cells at
-// or beyond F hold no particles; the clamps keep a table from a stage run
-// out of order from indexing past the arrays.
+// Particles of the half-cells [f, f + len) (Morton-adjacent, keys of the
+// batch), from the half-cell table ht; keys at or beyond F (the end of the
+// scene's keys) hold none of its particles. The clamps keep a table from a
+// stage run out of order from indexing past the arrays.
 __device__ inline void half_range(uint32_t f, uint32_t len, const uint32_t* __restrict__ ht,
                                   uint32_t F, uint32_t n, uint32_t& s, uint32_t& e) {
   if (f >= F) {
@@ -48,26 +46,27 @@ __device__ inline void half_range(uint32_t f, uint32_t len, const uint32_t* __re
   s = s < e ? s : e;
 }
 
-// Half-cell count usable for table lookups; 0 when the tables overflowed
-// (an error bit is raised by the grid-parameter kernel in that case).
-__device__ inline uint32_t usable_half_cells(const DevParams* __restrict__ dp) {
-  uint32_t G = dp->p.grid_cell_count;
-  return (G > dp->table_cap || (dp->err & kErrFineKey)) ? 0u : 8u * G;
+// End of the scene's keys usable for table lookups; 0 when the table
+// overflowed (an error bit is raised in that case).
+__device__ inline uint32_t usable_key_end(const DevParams* __restrict__ dp,
+                                          const BatchParams* __restrict__ bp) {
+  return (bp->err & (kErrTableCap | kErrFineKey)) ? 0u : dp->key_end;
 }
 
 // Visits the particles of the 3x3x3 half-cells around hc in z, y, x order.
 // Along x the three cells form two Morton-contiguous runs: the pair starting
 // at the even coordinate, and the remaining one.
 template <typename F>
-__device__ inline void for_each_candidate(const uint32_t hc[3], const uint32_t* __restrict__ ht,
-                                          uint32_t Fc, uint32_t n, F&& fn) {
+__device__ inline void for_each_candidate(const uint32_t hc[3], uint32_t base,
+                                          const uint32_t* __restrict__ ht, uint32_t Fc,
+                                          uint32_t n, F&& fn) {
   const uint32_t x0 = hc[0] - 1u;
   const bool even = (x0 & 1u) == 0u;
   for (uint32_t dz = 0; dz < 3; ++dz)
     for (uint32_t dy = 0; dy < 3; ++dy) {
       uint32_t y = hc[1] + dy - 1u, z = hc[2] + dz - 1u;
-      uint32_t a = half_key(x0, y, z);
-      uint32_t b = half_key(even ? x0 + 2u : x0 + 1u, y, z);
+      uint32_t a = base + half_key(x0, y, z);
+      uint32_t b = base + half_key(even ? x0 + 2u : x0 + 1u, y, z);
       uint32_t s, e;
       half_range(a, even ? 2u : 1u, ht, Fc, n, s, e);
       for (uint32_t j = s; j < e; ++j) fn(j);
@@ -80,28 +79,32 @@ __device__ inline void for_each_candidate(const uint32_t hc[3], const uint32_t* 
 // self excluded) in visiting order, four to a uint4, into
 // nbr[q * n + i] (q < kMaxNeighbours / 4), the count into nbr_count[i], and
 // the force inputs fa[i] = (x, p / rho^2) and fb[i] = (v, m / rho).
-__global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ pos,
-                                                    const float4* __restrict__ vel,
-                                                    const uint32_t* __restrict__ table,
-                                                    const DevParams* __restrict__ dp,
-                                                    float* __restrict__ rho,
-                                                    float* __restrict__ prs,
-                                                    float4* __restrict__ fa,
-                                                    float4* __restrict__ fb,
-                                                    uint4* __restrict__ nbr,
-                                                    uint32_t* __restrict__ nbr_count,
-                                                    uint32_t n) {
-  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+__global__ __launch_bounds__(kMaxBlock) void k_density(const float4* __restrict__ pos,
+                                                       const float4* __restrict__ vel,
+                                                       const uint32_t* __restrict__ table,
+                                                       const BlockDesc* __restrict__ blocks,
+                                                       const DevParams* __restrict__ dps,
+                                                       const BatchParams* __restrict__ bp,
+                                                       float* __restrict__ rho,
+                                                       float* __restrict__ prs,
+                                                       float4* __restrict__ fa,
+                                                       float4* __restrict__ fb,
+                                                       uint4* __restrict__ nbr,
+                                                       uint32_t* __restrict__ nbr_count,
+                                                       uint32_t n) {
+  uint32_t scene;
+  uint32_t i = particle_of(blocks, dps, scene);
+  if (i == 0xFFFFFFFFu) return;
+  const DevParams* dp = dps + scene;
   const float h = dp->p.h, h2 = h * h;
-  const uint32_t F = usable_half_cells(dp);
+  const uint32_t F = usable_key_end(dp, bp);
   float4 xi = pos[i];
   uint32_t hc[3];
   half_cell(xi, dp, hc);
   float acc = 0.f;
   uint32_t cnt = 0;
   uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;  // pending list entries
-  for_each_candidate(hc, table, F, n, [&](uint32_t j) {
+  for_each_candidate(hc, dp->key_base, table, F, n, [&](uint32_t j) {
     float4 xj = pos[j];
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float d2 = dx * dx + dy * dy + dz * dz;
@@ -182,16 +185,20 @@ __device__ inline void force_pair(ForceSums& f, const ForceConsts& k, float4 ai,
   f.lap += bj.w * k.c_lap * t * (3.f * k.h2 - 7.f * d2);
 }
 
-__global__ __launch_bounds__(kBlock) void k_forces(const float4* __restrict__ fa,
-                                                   const float4* __restrict__ fb,
-                                                   const float* __restrict__ rho,
-                                                   const uint32_t* __restrict__ table,
-                                                   const uint4* __restrict__ nbr,
-                                                   const uint32_t* __restrict__ nbr_count,
-                                                   const DevParams* __restrict__ dp,
-                                                   float4* __restrict__ acc_out, uint32_t n) {
-  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+__global__ __launch_bounds__(kMaxBlock) void k_forces(const float4* __restrict__ fa,
+                                                      const float4* __restrict__ fb,
+                                                      const float* __restrict__ rho,
+                                                      const uint32_t* __restrict__ table,
+                                                      const uint4* __restrict__ nbr,
+                                                      const uint32_t* __restrict__ nbr_count,
+                                                      const BlockDesc* __restrict__ blocks,
+                                                      const DevParams* __restrict__ dps,
+                                                      const BatchParams* __restrict__ bp,
+                                                      float4* __restrict__ acc_out, uint32_t n) {
+  uint32_t scene;
+  uint32_t i = particle_of(blocks, dps, scene);
+  if (i == 0xFFFFFFFFu) return;
+  const DevParams* dp = dps + scene;
   ForceConsts k;
   k.h = dp->p.h;
   k.h2 = k.h * k.h;
@@ -203,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(const float4* __restrict__ fa
   // Coincident-particle branch of spiky_gradient (smoothing.cl:23-25).
   const float h3 = k.h2 * k.h;
   k.spiky0 = -45.f / (3.14159274f * (h3 * h3));
-  const uint32_t F = usable_half_cells(dp);
+  const uint32_t F = usable_key_end(dp, bp);
   float4 ai = fa[i], bi = fb[i];
   float rho_i = rho[i];
   ForceSums fs;
@@ -236,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(const float4* __restrict__ fa
     uint32_t hc[3];
     half_cell(ai, dp, hc);
     const float h2 = k.h2;
-    for_each_candidate(hc, table, F, n, [&](uint32_t j) {
+    for_each_candidate(hc, dp->key_base, table, F, n, [&](uint32_t j) {
       float4 aj = fa[j];
       float dx = ai.x - aj.x, dy = ai.y - aj.y, dz = ai.z - aj.z;
       float d2 = dx * dx + dy * dy + dz * dz;
@@ -261,19 +268,19 @@ __global__ __launch_bounds__(kBlock) void k_forces(const float4* __restrict__ fa
   acc_out[i] = make_float4(f.x / rho_i + g.s[0], f.y / rho_i + g.s[1], f.z / rho_i + g.s[2], 0.f);
 }
 
-static inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
-
 void launch_density(clsph_state* s) {
   if (s->n == 0) return;
-  hipLaunchKernelGGL(k_density, dim3(blocks_for(s->n)), dim3(kBlock), 0, s->ctx->stream,
-                     s->pos_s, s->vel_s, s->table, s->dp, s->rho_s, s->prs_s, s->fa, s->fb,
+  hipLaunchKernelGGL(k_density, dim3(s->nblocks_of(kStageDensity)), dim3(s->bs_of(kStageDensity)),
+                     0, s->ctx->stream, s->pos_s, s->vel_s, s->table,
+                     s->table_of(kStageDensity), s->dps, s->bp, s->rho_s, s->prs_s, s->fa, s->fb,
                      s->nbr, s->nbr_count, s->n);
 }
 
 void launch_forces(clsph_state* s) {
   if (s->n == 0) return;
-  hipLaunchKernelGGL(k_forces, dim3(blocks_for(s->n)), dim3(kBlock), 0, s->ctx->stream, s->fa,
-                     s->fb, s->rho_s, s->table, s->nbr, s->nbr_count, s->dp, s->acc_s, s->n);
+  hipLaunchKernelGGL(k_forces, dim3(s->nblocks_of(kStageForces)), dim3(s->bs_of(kStageForces)), 0,
+                     s->ctx->stream, s->fa, s->fb, s->rho_s, s->table, s->nbr, s->nbr_count,
+                     s->table_of(kStageForces), s->dps, s->bp, s->acc_s, s->n);
 }
 
 }  // namespace clsph

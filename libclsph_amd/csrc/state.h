@@ -1,7 +1,15 @@
 // Host-side view of a simulation's device state and the stage launchers.
+//
+// A state holds one or more independent scenes (a batch). Scene s owns the
+// particles [begin_s, begin_s + n_s) of every particle array, in every
+// order; its half-cell keys occupy [key_base_s, key_base_s + 8 G_s) of one
+// shared key space, so the sort, the tables and the canonical slots serve
+// all scenes at once. Per-particle kernels run over a table of work-groups,
+// each inside one scene.
 #pragma once
 
 #include <string>
+#include <vector>
 
 #include "common.h"
 
@@ -12,11 +20,18 @@ struct clsph_ctx {
   int cus = 256;  // compute units on the device
 };
 
+// Stage indices (clsph_time_stage order).
+enum { kStageGrid, kStageHash, kStageSort, kStageTable, kStageDensity, kStageForces,
+       kStageAdvect, kNumStages };
+
 struct clsph_state {
   clsph_ctx* ctx = nullptr;
-  uint32_t n_cap = 0;
-  uint32_t n = 0;
-  uint32_t table_cap = 0;
+  uint32_t n_scenes = 1;
+  bool fixed_counts = false;  // batches keep the per-scene counts given at allocation
+  uint32_t n_cap = 0;         // particles the arrays hold
+  uint32_t n = 0;             // particles in all scenes
+  std::vector<uint32_t> scene_begin, scene_n, scene_cap;
+  uint32_t table_cap = 0;     // half-cell keys the table holds (all scenes)
 
   // Canonical particle arrays, in the order the reference's host array holds
   // them at the start of a substep (the previous substep's sorted order).
@@ -24,7 +39,7 @@ struct clsph_state {
   float4* vel = nullptr;  // velocity
   float4* vh = nullptr;   // intermediate_velocity
 
-  // Sorted working arrays for the current substep.
+  // Sorted working arrays for the current substep (half-cell order).
   float4* pos_s = nullptr;
   float4* vel_s = nullptr;
   float4* vh_s = nullptr;
@@ -42,35 +57,45 @@ struct clsph_state {
   uint32_t* val_a = nullptr;
   uint32_t* key_b = nullptr;
   uint32_t* val_b = nullptr;
-  uint32_t* hist = nullptr;      // per-tile digit counts, digit-major
-  uint32_t* digit_base = nullptr;  // exclusive prefix of digit totals
+  uint32_t* hist = nullptr;        // per-tile digit counts, digit-major
+  uint32_t* digit_base = nullptr;  // digit totals
   uint32_t n_tiles_cap = 0;
 
-  // Half-cell table (8 * table_cap + 1 entries): table[f] = #{keys < f}; the
-  // reference's cell table is table[8c]. slot[k]: the canonical position
-  // (stable order by cell id) of the particle at sorted position k.
+  // Half-cell table (table_cap + 1 entries): table[f] = #{keys < f}; the
+  // reference's cell table of scene s is table[key_base_s + 8c] - begin_s.
+  // slot[k]: the canonical position (stable order by cell id) of the
+  // particle at sorted position k.
   uint32_t* table = nullptr;
   uint32_t* slot = nullptr;
 
-  // Per-block bounding boxes (6 floats per 256-particle block) written by the
-  // bounding-box kernel or by advection, reduced by the grid-parameter kernel.
+  // Per-work-group bounding boxes (6 floats each) written by the bounding-box
+  // kernel or by advection over the advection work-group table, reduced per
+  // scene by the grid-parameter kernel.
   float* aabb_partial = nullptr;
 
-  // Collision mesh (uploaded once, sph_simulation.cpp:296-319 re-uploads it
-  // every substep).
+  // Work-group tables, one per block size (kBlockSizes), and the block size
+  // each stage runs with. Advection and the bounding-box kernel share one.
+  clsph::BlockDesc* blocks[clsph::kNumBlockSizes] = {};
+  uint32_t nblocks[clsph::kNumBlockSizes] = {};
+  int stage_bs[kNumStages] = {2, 2, 2, 2, 2, 2, 2};  // indices into kBlockSizes
+
+  // Collision meshes of all scenes, concatenated (uploaded once;
+  // sph_simulation.cpp:296-319 re-uploads it every substep).
   float* face_normals = nullptr;
   float* vertices = nullptr;
   uint32_t* indices = nullptr;
-  uint32_t face_count = 0;
-  uint32_t vert_floats = 0;
-  // Uniform triangle grid (CSR: tgrid_start[cell]..tgrid_start[cell+1] index
-  // tgrid_tris), see clsph_scene_grid.
-  clsph::TriGrid tgrid{};
-  uint32_t* tgrid_start = nullptr;
+  uint32_t* tgrid_start = nullptr;  // triangle-grid cells of all scenes (CSR)
   uint32_t* tgrid_tris = nullptr;
+  struct Mesh {
+    std::vector<float> normals, vertices;
+    std::vector<uint32_t> indices;
+    float grid_cell = -1.f;  // < 0: no grid; 0: automatic; > 0: cell size
+  };
+  std::vector<Mesh> meshes;
 
-  clsph::DevParams* dp = nullptr;  // device parameter block
-  clsph::DevParams host_dp{};      // host copy of the host-set fields
+  clsph::DevParams* dps = nullptr;   // device parameters, one per scene
+  clsph::BatchParams* bp = nullptr;  // device batch parameters
+  std::vector<clsph::DevParams> host_dps;
 
   hipGraphExec_t graph = nullptr;  // one captured substep
   bool graph_dirty = true;
@@ -82,6 +107,10 @@ struct clsph_state {
   // True while rho_s / prs_s / key_b / acc_s still hold the uploaded
   // particles in canonical order (no substep has run since the upload).
   bool fields_canonical = true;
+
+  int bs_of(int stage_index) const { return clsph::kBlockSizes[stage_bs[stage_index]]; }
+  const clsph::BlockDesc* table_of(int stage_index) const { return blocks[stage_bs[stage_index]]; }
+  uint32_t nblocks_of(int stage_index) const { return nblocks[stage_bs[stage_index]]; }
 };
 
 namespace clsph {

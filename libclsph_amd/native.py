@@ -71,6 +71,15 @@ def _load():
         "clsph_kernel_terms": (None, [ctypes.c_float, _vp]),
         "clsph_make_box": (None, [_vp, _vp, _vp, _vp, _vp]),
         "clsph_scene_grid": (_i32, [_vp, ctypes.c_float]),
+        "clsph_batch_alloc": (_i32, [_vp, _u32, _vp, _u32, ctypes.POINTER(_vp)]),
+        "clsph_batch_set_params": (_i32, [_vp, _u32, _vp, _vp, _u32]),
+        "clsph_batch_upload": (_i32, [_vp, _u32, _vp, _u32]),
+        "clsph_batch_download": (_i32, [_vp, _u32, _vp, _u32]),
+        "clsph_batch_scene_upload": (_i32, [_vp, _u32, _vp, _vp, _u32, _vp, _u32]),
+        "clsph_batch_get_grid": (_i32, [_vp, _u32, _vp]),
+        "clsph_batch_bounce_capped": (_i32, [_vp, _u32, _vp]),
+        "clsph_set_block_sizes": (_i32, [_vp, _vp]),
+        "clsph_get_block_sizes": (_i32, [_vp, _vp]),
         "clsph_make_heightfield": (None, [_u32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                           ctypes.c_float, _u32, ctypes.c_uint64, _vp, _vp, _vp]),
         "clsph_make_column": (_u32, [_vp, _u32, ctypes.c_float, _vp, _vp, ctypes.c_float,
@@ -199,6 +208,16 @@ class State:
     def set_params(self, params, terms, max_bounces=0):
         self.ctx.check(lib.clsph_set_params(self.h, _ptr(params), _ptr(terms), max_bounces))
 
+    def block_sizes(self):
+        b = np.zeros(7, dtype=np.uint32)
+        self.ctx.check(lib.clsph_get_block_sizes(self.h, _ptr(b)))
+        return [int(v) for v in b]
+
+    def set_block_sizes(self, sizes):
+        b = np.asarray(sizes, dtype=np.uint32)
+        assert b.shape == (7,)
+        self.ctx.check(lib.clsph_set_block_sizes(self.h, _ptr(b)))
+
     def upload(self, parts):
         parts = np.ascontiguousarray(parts, dtype=layout.PARTICLE)
         self.ctx.check(lib.clsph_upload_aos(self.h, _ptr(parts), len(parts)))
@@ -276,3 +295,49 @@ class State:
         ms = np.zeros(1, dtype=np.float32)
         self.ctx.check(lib.clsph_time_stage(self.h, STAGES[name], reps, _ptr(ms)))
         return float(ms[0])
+
+
+class Batch(State):
+    """Independent scenes of fixed particle counts advanced together: every
+    stage runs as one launch over all scenes."""
+
+    def __init__(self, ctx, counts, table_cap=0):
+        self.ctx = ctx
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        h = ctypes.c_void_p()
+        ctx.check(lib.clsph_batch_alloc(ctx.h, len(counts), _ptr(counts), table_cap,
+                                        ctypes.byref(h)))
+        self.h = h
+        self.counts = [int(c) for c in counts]
+        self.n = int(counts.sum())
+
+    def set_scene_params(self, scene, params, terms, max_bounces=0):
+        self.ctx.check(lib.clsph_batch_set_params(self.h, scene, _ptr(params), _ptr(terms),
+                                                  max_bounces))
+
+    def upload_scene(self, scene, parts):
+        parts = np.ascontiguousarray(parts, dtype=layout.PARTICLE)
+        self.ctx.check(lib.clsph_batch_upload(self.h, scene, _ptr(parts), len(parts)))
+
+    def download_scene(self, scene):
+        out = layout.particles(self.counts[scene])
+        self.ctx.check(lib.clsph_batch_download(self.h, scene, _ptr(out), len(out)))
+        return out
+
+    def mesh_scene(self, scene, normals, vertices, indices):
+        normals = np.ascontiguousarray(normals, dtype=np.float32)
+        vertices = np.ascontiguousarray(vertices, dtype=np.float32)
+        indices = np.ascontiguousarray(indices, dtype=np.uint32)
+        self.ctx.check(lib.clsph_batch_scene_upload(self.h, scene, _ptr(normals), _ptr(vertices),
+                                                    len(vertices) // 3, _ptr(indices),
+                                                    len(indices) // 3))
+
+    def grid_scene(self, scene):
+        p = layout.params()
+        self.ctx.check(lib.clsph_batch_get_grid(self.h, scene, _ptr(p)))
+        return p
+
+    def bounce_capped_scene(self, scene):
+        c = np.zeros(1, dtype=np.uint32)
+        self.ctx.check(lib.clsph_batch_bounce_capped(self.h, scene, _ptr(c)))
+        return int(c[0])

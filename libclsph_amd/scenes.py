@@ -57,11 +57,20 @@ def fluid_params(n, h, mass, dt=1e-4, viscosity=3.5e-3, stiffness=3.0, rest_dens
     return p
 
 
-def dam_break(n, seed=1234, jitter=0.01, block=(0.4, 0.8, 0.4), box=1.0, **kw):
-    spacing = np.float32((0.128 / n) ** (1.0 / 3.0))
-    if block != (0.4, 0.8, 0.4):
-        vol = block[0] * block[1] * block[2]
-        spacing = np.float32((vol / n) ** (1.0 / 3.0))
+def dam_break(n, seed=1234, jitter=0.01, block=(0.4, 0.8, 0.4), box=1.0, spacing=None, **kw):
+    """The dam break with n particles. By default the lattice spacing makes n
+    particles fill the block; with `spacing` given, the block and box are
+    scaled by cbrt(n s^3 / 0.128) instead, keeping that spacing."""
+    if spacing is not None:
+        spacing = np.float32(spacing)
+        k = (n * float(spacing) ** 3 / 0.128) ** (1.0 / 3.0)
+        block = (0.4 * k, 0.8 * k, 0.4 * k)
+        box = box * k
+    else:
+        spacing = np.float32((0.128 / n) ** (1.0 / 3.0))
+        if block != (0.4, 0.8, 0.4):
+            vol = block[0] * block[1] * block[2]
+            spacing = np.float32((vol / n) ** (1.0 / 3.0))
     h = np.float32(2.0) * spacing
     mass = np.float32(1000.0 * float(spacing) ** 3)
     parts = native.make_block(n, float(spacing), (0.0, 0.0, 0.0), block, jitter, seed)
@@ -117,3 +126,17 @@ def heightfield_pour(n, seed=4321, column=(0.3, 0.3), base=0.15, dt=5e-5, **kw):
     terms = native.kernel_terms(float(h))
     normals, verts, idx = terrain()
     return Scene(parts, params, terms, normals, verts, idx)
+
+
+def dam_break_batch(n_scenes=64, lo=4096, hi=262144, seed=99, spacing=None):
+    """Independent dam breaks with particle counts log-uniform in [lo, hi] and
+    dynamic viscosity uniform in [1e-3, 5e-2] (seeded). Every scene keeps the
+    lattice spacing of the 1,048,576-particle dam break (spacing 0.5 h), its
+    block and box scaled to its count."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(lo), np.log(hi), n_scenes)).astype(np.int64)
+    visc = rng.uniform(1e-3, 5e-2, n_scenes)
+    if spacing is None:
+        spacing = (0.128 / 1048576) ** (1.0 / 3.0)
+    return [dam_break(int(c), seed=seed * 1000 + i, spacing=spacing, viscosity=float(v))
+            for i, (c, v) in enumerate(zip(counts, visc))]
