@@ -169,6 +169,78 @@ int clsph_time_stage(clsph_state* st, int stage, uint32_t reps, float* ms);
  * clsph_time_stage; stage 2 includes the state gather into sorted order). */
 int clsph_profile_steps(clsph_state* st, uint32_t nsteps, float* stage_ms);
 
+/* ---- learned force surrogate ----
+ * No counterpart in the reference, whose only acceleration model is the
+ * `forces` kernel (kernels/sph.cl:42-62). A two-round message-passing
+ * network over the neighbour graph of the density stage (r < h, at most 128
+ * neighbours per particle) predicts each particle's acceleration; its
+ * products run on bf16 matrix cores with float32 accumulation. Parameters
+ * are one float32 vector of clsph_nn_param_count() values (row-major
+ * inputs x outputs matrices and biases in a fixed order). Single-scene
+ * states only. */
+typedef struct clsph_nn clsph_nn;
+uint32_t clsph_nn_param_count(void);
+/* Workspace for states of up to n_cap particles and graphs of up to e_cap
+ * edges (0: 64 n_cap); parameters start at zero. */
+int clsph_nn_create(clsph_ctx* ctx, uint32_t n_cap, uint32_t e_cap, clsph_nn** out);
+int clsph_nn_free(clsph_nn* nn);
+/* Sets the parameters (and clears the optimiser moments) / reads them. */
+int clsph_nn_set_params(clsph_nn* nn, const float* params, uint32_t count);
+int clsph_nn_get_params(clsph_nn* nn, float* params, uint32_t count);
+/* Evaluates the network on a state whose last stage was density: builds the
+ * graph, writes the accelerations where the forces stage would (the
+ * following advection uses them) and the edge count to *edges (optional). */
+int clsph_nn_forward(clsph_nn* nn, clsph_state* st, uint32_t* edges);
+/* The same evaluation with the fused inference kernels: no per-edge
+ * storage (any edge count), no host synchronisation, no gradients. */
+int clsph_nn_infer(clsph_nn* nn, clsph_state* st);
+/* Runs clsph_nn_infer `reps` times with HIP events on the context stream;
+ * ms[0..3] = average milliseconds of edge round 1, the node part of round 1,
+ * edge round 2 and the node part of round 2; *edges (optional) = the edges
+ * evaluated per pass. */
+int clsph_nn_profile(clsph_nn* nn, clsph_state* st, uint32_t reps, float* ms, uint64_t* edges);
+/* The graph of the last evaluation (dst-major CSR: off[n + 1], src[e]) and
+ * the raw network output (n x 32 floats; columns 0..2 are (a - g) / 10). */
+int clsph_nn_read_graph(clsph_nn* nn, uint32_t* off, uint32_t* src, uint32_t n, uint32_t e);
+int clsph_nn_read_output(clsph_nn* nn, float* out, uint32_t n);
+/* Training target (n x 3, in the output's units) from the host, or from two
+ * states: `learner` sorted for the current substep and `teacher` started as
+ * a copy of the learner's particles with identities tracked, then advanced
+ * by the plain method over `dt` seconds. */
+int clsph_nn_set_target(clsph_nn* nn, const float* target, uint32_t n);
+int clsph_nn_read_target(clsph_nn* nn, float* target, uint32_t n);
+int clsph_nn_target_from(clsph_nn* nn, clsph_state* learner, clsph_state* teacher, float dt);
+/* Mean squared error of the last evaluation against the target and its
+ * gradient with respect to the parameters (*loss optional), then one Adam
+ * step (bias-corrected). */
+int clsph_nn_backward(clsph_nn* nn, float* loss);
+int clsph_nn_read_grad(clsph_nn* nn, float* grad, uint32_t count);
+int clsph_nn_adam(clsph_nn* nn, float lr, float beta1, float beta2, float eps);
+/* `nsteps` substeps with the network (fused inference kernels) in place of
+ * the forces stage, at the state's time_delta. */
+int clsph_learned_step(clsph_state* st, clsph_nn* nn, uint32_t nsteps);
+/* Density difference of the same particles in two states after both ran
+ * their density stage (identities tracked in both): max and mean of
+ * |rho_a - rho_b| / fluid_density. */
+int clsph_nn_compare_density(clsph_nn* nn, clsph_state* a, clsph_state* b, float* max_rel,
+                             float* mean_rel);
+/* Copies the particles (and the per-particle fields) of single-scene `src`
+ * into `dst`, and their identities when both track them; parameters and
+ * meshes of `dst` stay its own. */
+int clsph_state_copy(clsph_state* dst, clsph_state* src);
+/* Particle identities: enable resets them to the current canonical index
+ * (0..n-1) and keeps them up to date through every substep; 0 disables. */
+int clsph_track_ids(clsph_state* st, int enable);
+int clsph_read_ids(clsph_state* st, uint32_t* ids, uint32_t n);
+/* The network's bf16 matrix-core product on host arrays (for testing):
+ * c (m x n float32, row-major) = op(a) op(b) [+ bias], or += with
+ * `accumulate`. a is m x k bf16 (k x m with trans_a), b is k x n (n x k
+ * with trans_b), bf16 given as raw 16-bit patterns; the contiguous
+ * dimension of each operand must be a multiple of 8. */
+int clsph_nn_gemm(clsph_ctx* ctx, int trans_a, int trans_b, int accumulate, const uint16_t* a,
+                  const uint16_t* b, float* c, const float* bias, uint32_t m, uint32_t n,
+                  uint32_t k);
+
 /* ---- scene builders (host only; no device needed) ---- */
 /* Dam-break block: a cubic lattice of `n` particles at `spacing` with
  * round(extent/spacing) sites along x and z, stacked layer by layer from

@@ -9,56 +9,13 @@
 #include <vector>
 
 #include "../../include/clsph_hip.h"
+#include "hostutil.h"
 #include "state.h"
 
 using namespace clsph;
+using namespace clsph::host;
 
 namespace {
-
-thread_local std::string g_err;  // errors raised before a context exists
-
-int fail(clsph_ctx* ctx, int code, const std::string& msg) {
-  if (ctx)
-    ctx->last_error = msg;
-  else
-    g_err = msg;
-  return code;
-}
-
-#define HIPCK(ctx, expr)                                                        \
-  do {                                                                          \
-    hipError_t e_ = (expr);                                                     \
-    if (e_ != hipSuccess)                                                       \
-      return fail((ctx), CLSPH_EDEVICE,                                         \
-                  std::string(#expr) + ": " + hipGetErrorString(e_));           \
-  } while (0)
-
-template <typename T>
-int dalloc(clsph_ctx* ctx, T** p, size_t count) {
-  *p = nullptr;
-  if (count == 0) count = 1;
-  hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return fail(ctx, CLSPH_ENOMEM,
-                std::string("hipMalloc of ") + std::to_string(sizeof(T) * count) +
-                    " bytes: " + hipGetErrorString(e));
-  }
-  return CLSPH_OK;
-}
-
-template <typename T>
-void dfree(T*& p) {
-  if (p) (void)hipFree(p);
-  p = nullptr;
-}
-
-int check_launch(clsph_ctx* ctx) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess)
-    return fail(ctx, CLSPH_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
-  return CLSPH_OK;
-}
 
 int read_dps(clsph_state* st, std::vector<DevParams>& out, BatchParams* bp = nullptr) {
   out.resize(st->n_scenes);
@@ -70,23 +27,6 @@ int read_dps(clsph_state* st, std::vector<DevParams>& out, BatchParams* bp = nul
   return CLSPH_OK;
 }
 
-int device_errors(clsph_state* st) {
-  std::vector<DevParams> d;
-  BatchParams b;
-  int rc = read_dps(st, d, &b);
-  if (rc) return rc;
-  uint32_t err = b.err;
-  for (const DevParams& x : d) err |= x.err;
-  if (err & kErrGridDim)
-    return fail(st->ctx, CLSPH_EGRID,
-                "grid dimension >= 1024 cells on an axis (particles spread too far)");
-  if (err & kErrTableCap)
-    return fail(st->ctx, CLSPH_ETABLE, "grid cells exceed the cell table capacity");
-  if (err & kErrFineKey)
-    return fail(st->ctx, CLSPH_ETABLE, "grid_cell_count reached 2^29 cells");
-  return CLSPH_OK;
-}
-
 void free_state_buffers(clsph_state* st) {
   dfree(st->pos), dfree(st->vel), dfree(st->vh);
   dfree(st->pos_s), dfree(st->vel_s), dfree(st->vh_s), dfree(st->acc_s);
@@ -94,6 +34,7 @@ void free_state_buffers(clsph_state* st) {
   dfree(st->nbr), dfree(st->nbr_count);
   dfree(st->key_a), dfree(st->val_a), dfree(st->key_b), dfree(st->val_b);
   dfree(st->hist), dfree(st->digit_base), dfree(st->table), dfree(st->slot);
+  dfree(st->ids), dfree(st->ids_tmp);
   dfree(st->aabb_partial);
   dfree(st->face_normals), dfree(st->vertices), dfree(st->indices);
   dfree(st->tgrid_start), dfree(st->tgrid_tris);
@@ -297,6 +238,7 @@ void enqueue_step(clsph_state* st) {
   launch_density(st);
   launch_forces(st);
   launch_advect(st);
+  if (st->ids) launch_track_ids(st);
 }
 
 int run_stage(clsph_state* st, int stage) {
@@ -326,6 +268,7 @@ int run_stage(clsph_state* st, int stage) {
       break;
     case kStageAdvect:
       launch_advect(st);
+      if (st->ids) launch_track_ids(st);
       st->acc_zero = true;
       break;
     default:
@@ -592,6 +535,39 @@ int ensure_graph(clsph_state* st) {
 }
 
 }  // namespace
+
+namespace clsph {
+namespace host {
+
+int device_errors(clsph_state* st) {
+  std::vector<DevParams> d;
+  BatchParams b;
+  int rc = read_dps(st, d, &b);
+  if (rc) return rc;
+  uint32_t err = b.err;
+  for (const DevParams& x : d) err |= x.err;
+  if (err & kErrGridDim)
+    return fail(st->ctx, CLSPH_EGRID,
+                "grid dimension >= 1024 cells on an axis (particles spread too far)");
+  if (err & kErrTableCap)
+    return fail(st->ctx, CLSPH_ETABLE, "grid cells exceed the cell table capacity");
+  if (err & kErrFineKey)
+    return fail(st->ctx, CLSPH_ETABLE, "grid_cell_count reached 2^29 cells");
+  return CLSPH_OK;
+}
+
+int resize_single(clsph_state* st, uint32_t n) {
+  if (st->n_scenes != 1 || st->fixed_counts || n > st->scene_cap[0])
+    return fail(st->ctx, CLSPH_EINVAL, "only a single-scene state can change its count");
+  st->scene_n[0] = n;
+  st->n = n;
+  int rc = build_blocks(st);
+  if (!rc) rc = upload_params(st);
+  return rc;
+}
+
+}  // namespace host
+}  // namespace clsph
 
 extern "C" {
 

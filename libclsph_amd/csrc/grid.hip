@@ -270,3 +270,31 @@ void launch_cell_table(clsph_state* s) {
 }
 
 }  // namespace clsph
+
+// Identity tracking across a substep: the particle at sorted position k came
+// from canonical index perm[k] and is written back to canonical slot[k].
+__global__ __launch_bounds__(256) void k_track_ids(const uint32_t* __restrict__ perm,
+                                                   const uint32_t* __restrict__ slot,
+                                                   const uint32_t* __restrict__ ids,
+                                                   uint32_t* __restrict__ out, uint32_t n) {
+  uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  uint32_t s = slot[k], p = perm[k];
+  if (s < n && p < n) out[s] = ids[p];
+}
+
+__global__ __launch_bounds__(256) void k_copy_u32(const uint32_t* __restrict__ in,
+                                                  uint32_t* __restrict__ out, uint32_t n) {
+  uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k < n) out[k] = in[k];
+}
+
+namespace clsph {
+void launch_track_ids(clsph_state* s) {
+  if (!s->n) return;
+  dim3 g((s->n + 255) / 256);
+  hipLaunchKernelGGL(k_track_ids, g, dim3(256), 0, s->ctx->stream, s->val_b, s->slot, s->ids,
+                     s->ids_tmp, s->n);
+  hipLaunchKernelGGL(k_copy_u32, g, dim3(256), 0, s->ctx->stream, s->ids_tmp, s->ids, s->n);
+}
+}  // namespace clsph

@@ -68,6 +68,11 @@ struct clsph_state {
   uint32_t* table = nullptr;
   uint32_t* slot = nullptr;
 
+  // Optional particle identities in canonical order (clsph_track_ids): after
+  // each substep ids[slot[k]] takes the id of the particle sorted to k.
+  uint32_t* ids = nullptr;
+  uint32_t* ids_tmp = nullptr;
+
   // Per-work-group bounding boxes (6 floats each) written by the bounding-box
   // kernel or by advection over the advection work-group table, reduced per
   // scene by the grid-parameter kernel.
@@ -121,6 +126,7 @@ void launch_grid_params(clsph_state* s);
 void launch_hash(clsph_state* s);
 void launch_permute(clsph_state* s);
 void launch_cell_table(clsph_state* s);
+void launch_track_ids(clsph_state* s);
 // sort.hip
 void launch_sort(clsph_state* s);
 uint32_t sort_tiles_for(uint32_t n);

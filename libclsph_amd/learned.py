@@ -1,0 +1,227 @@
+"""Training and running the learned force surrogate.
+
+The surrogate (the network of libclsph_amd/csrc/nn.h) predicts each
+particle's acceleration over one macro step of 4 substeps, so that the
+learned method advances at dt_macro = 4 dt with one neighbour search, one
+network evaluation and one advection per macro step.
+
+Training data comes from the plain GPU solver, generated online: each
+trajectory is a seeded dam-break variant, advanced by the plain substep;
+every `gap` substeps the current state is copied into a learner state,
+sorted and given its densities, and the teacher's next 4 plain substeps
+(with particle identities tracked) give each particle's target
+
+    a*_i = (v_half_i(t + 4 dt) - v_half_i(t)) / (4 dt)
+
+(the acceleration that makes one leapfrog step of 4 dt reproduce the
+plain method's half-step velocity). Each sample is one forward, one
+backward and one Adam step, all on the GPU.
+
+Running (run_learned): macro steps with the network in place of the forces
+stage. Every `judge_every` macro steps the plain method is the judge: the
+same start is advanced by 4 plain substeps, both states get their densities,
+and when max_i |rho_i - rho_i^plain| / rho0 exceeds `tol` (1 %), the macro
+step is replaced by the plain result (a fallback).
+
+    python -m libclsph_amd.learned --out weights.npy --trajectories 200 --minutes 8
+"""
+import argparse
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import native, scenes
+
+MACRO = 4
+SPACING_1M = (0.128 / 1048576) ** (1.0 / 3.0)
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",
+                       "surrogate_mi355x.npy")
+DENSITY_STAGES = ("grid", "hash", "sort", "cell_table", "density")
+
+# Parameter vector layout of the network (libclsph_amd/csrc/nn.h, enum Param):
+# (name, shape, used input rows) in storage order, row-major inputs x outputs.
+W, FEAT, IN1, IN2, OUT = 128, 16, 144, 256, 32
+LAYOUT = [
+    ("e1.w1", (FEAT, W), 12), ("e1.b1", (W,), 0), ("e1.w2", (W, W), W), ("e1.b2", (W,), 0),
+    ("e1.w3", (W, W), W), ("e1.b3", (W,), 0),
+    ("n1.w1", (IN1, W), W + 2), ("n1.b1", (W,), 0), ("n1.w2", (W, W), W), ("n1.b2", (W,), 0),
+    ("n1.w3", (W, W), W), ("n1.b3", (W,), 0),
+    ("e2.w1", (FEAT, W), 12), ("e2.b1", (W,), 0), ("e2.wa", (W, W), W), ("e2.wb", (W, W), W),
+    ("e2.w2", (W, W), W), ("e2.b2", (W,), 0), ("e2.w3", (W, W), W), ("e2.b3", (W,), 0),
+    ("n2.w1", (IN2, W), IN2), ("n2.b1", (W,), 0), ("n2.w2", (W, W), W), ("n2.b2", (W,), 0),
+    ("n2.w3", (W, OUT), W), ("n2.b3", (OUT,), 0),
+]
+
+
+def init_params(seed=0):
+    """Initial parameters: weights U(-1, 1) / sqrt(fan_in) over the used
+    input rows (padded rows zero), biases zero, and the output layer scaled
+    by 0.1 with only its first 3 columns nonzero."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for name, shape, used in LAYOUT:
+        if len(shape) == 1:
+            parts.append(np.zeros(shape))
+            continue
+        w = rng.uniform(-1.0, 1.0, size=shape) / np.sqrt(used)
+        w[used:] = 0.0
+        if name == "n2.w3":
+            w[:, 3:] = 0.0
+            w *= 0.1
+        parts.append(w)
+    return np.concatenate([p.ravel() for p in parts]).astype(np.float32)
+
+
+def slab_variant(seed, spacing=SPACING_1M, depth_sites=6):
+    """A dam-break variant at the spacing of the 1,048,576-particle dam
+    break: block b x 2b x (depth_sites spacings) with b ~ U(0.3, 0.5) m, in a
+    2.5 b x 2.5 b box of the same depth; jitter ~ U(0, 0.02) spacings and
+    dynamic viscosity ~ U(1e-3, 1e-2)."""
+    rng = np.random.default_rng(seed)
+    b = float(rng.uniform(0.3, 0.5))
+    s = np.float32(spacing)
+    nx = int(round(b / float(s)))
+    ny = int(round(2.0 * b / float(s)))
+    nz = int(depth_sites)
+    n = nx * ny * nz
+    depth = nz * float(s)
+    jitter = float(rng.uniform(0.0, 0.02))
+    visc = float(rng.uniform(1e-3, 1e-2))
+    h = np.float32(2.0) * s
+    mass = np.float32(1000.0 * float(s) ** 3)
+    parts = native.make_block(n, float(s), (0.0, 0.0, 0.0), (nx * float(s), 0.0, depth),
+                              jitter, int(seed) * 7919 + 11)
+    params = scenes.fluid_params(n, h, mass, viscosity=visc)
+    terms = native.kernel_terms(float(h))
+    normals, verts, idx = native.make_box((0.0, 0.0, 0.0), (2.5 * b, 2.5 * b, depth))
+    return scenes.Scene(parts, params, terms, normals, verts, idx)
+
+
+def make_state(ctx, sc, n_cap=None, dt=None):
+    st = native.State(ctx, n_cap or sc.n)
+    p = sc.params.copy()
+    if dt is not None:
+        p["time_delta"] = np.float32(dt)
+    st.set_params(p, sc.terms)
+    st.scene(sc.normals, sc.vertices, sc.indices)
+    return st
+
+
+def density_stages(st):
+    for name in DENSITY_STAGES:
+        st.stage(name)
+
+
+def train(net, ctx, trajectories=200, steps=2000, gap=20, minutes=None, lr=1e-3, lr_end=1e-5,
+          seed=0, log=print, warm=50):
+    """Online training over plain-solver trajectories; returns the loss
+    history. Stops early after `minutes` of wall-clock time."""
+    t0 = time.perf_counter()
+    samples_per_traj = max(1, (steps - warm) // gap)
+    total = trajectories * samples_per_traj
+    losses = []
+    k = 0
+    for tr in range(trajectories):
+        sc = slab_variant(seed + tr)
+        teacher = make_state(ctx, sc)
+        learner = make_state(ctx, sc)
+        teacher.upload(sc.particles)
+        teacher.track_ids(True)
+        teacher.step(warm)
+        dt = float(sc.params["time_delta"])
+        for smp in range(samples_per_traj):
+            learner.copy_from(teacher)
+            density_stages(learner)
+            teacher.track_ids(True)
+            teacher.step(MACRO)
+            net.forward(learner)
+            net.target_from(learner, teacher, MACRO * dt)
+            frac = min(1.0, k / max(1, total - 1))
+            rate = lr_end + 0.5 * (lr - lr_end) * (1.0 + math.cos(math.pi * frac))
+            loss = net.backward(read_loss=(smp % 10 == 0))
+            net.adam(rate)
+            if loss is not None:
+                losses.append((k, loss))
+            k += 1
+            teacher.step(gap - MACRO)
+        teacher.close()
+        learner.close()
+        last = losses[-1][1] if losses else float("nan")
+        log(f"trajectory {tr + 1}/{trajectories} (n={sc.n}): {k} samples, loss {last:.4e}, "
+            f"{time.perf_counter() - t0:.0f} s")
+        if minutes is not None and time.perf_counter() - t0 > 60.0 * minutes:
+            break
+    return losses
+
+
+class Run:
+    """Outcome of run_learned."""
+
+    def __init__(self):
+        self.macro_steps = 0
+        self.judged = 0
+        self.fallbacks = 0
+        self.judge_err = []
+        self.wall_s = 0.0
+        self.sim_s = 0.0
+
+
+def run_learned(ctx, learner, teacher, net, macro_steps, dt, judge_every=8, tol=0.01):
+    """Advances `learner` (parameters at dt_macro = 4 dt) by `macro_steps`
+    macro steps with the network, judged every `judge_every` macro steps by
+    4 plain substeps of `teacher` (parameters at dt) from the same start.
+    Both states must track identities."""
+    r = Run()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for k in range(macro_steps):
+        if judge_every and k % judge_every == 0:
+            teacher.copy_from(learner)
+            learner.learned_step(net, 1)
+            teacher.step(MACRO)
+            density_stages(learner)
+            density_stages(teacher)
+            mx, _ = net.compare_density(learner, teacher)
+            r.judged += 1
+            r.judge_err.append(mx)
+            if mx > tol:
+                learner.copy_from(teacher)
+                r.fallbacks += 1
+        else:
+            learner.learned_step(net, 1)
+        r.macro_steps += 1
+    ctx.sync()
+    r.wall_s = time.perf_counter() - t0
+    r.sim_s = macro_steps * MACRO * dt
+    return r
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--trajectories", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--gap", type=int, default=20)
+    ap.add_argument("--minutes", type=float, default=None)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--init", default=None, help="start from these weights")
+    a = ap.parse_args()
+    ctx = native.Context(0)
+    n_cap = 100 * 200 * 6  # largest slab variant: 100 x 200 x 6 sites
+    net = native.Network(ctx, n_cap, e_cap=96 * n_cap)
+    net.set_params(np.load(a.init) if a.init else init_params(a.seed))
+    losses = train(net, ctx, trajectories=a.trajectories, steps=a.steps, gap=a.gap,
+                   minutes=a.minutes, lr=a.lr, seed=a.seed,
+                   log=lambda m: print(m, flush=True))
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    np.save(a.out, net.params())
+    np.save(os.path.splitext(a.out)[0] + "_loss.npy", np.asarray(losses, dtype=np.float64))
+    print(f"wrote {a.out}", file=sys.stderr)
+
+
+if __name__ == "__main__":
+    main()

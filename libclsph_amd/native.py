@@ -85,6 +85,29 @@ def _load():
         "clsph_make_column": (_u32, [_vp, _u32, ctypes.c_float, _vp, _vp, ctypes.c_float,
                                      ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                      ctypes.c_uint64]),
+        "clsph_nn_param_count": (_u32, []),
+        "clsph_nn_create": (_i32, [_vp, _u32, _u32, ctypes.POINTER(_vp)]),
+        "clsph_nn_free": (_i32, [_vp]),
+        "clsph_nn_set_params": (_i32, [_vp, _vp, _u32]),
+        "clsph_nn_get_params": (_i32, [_vp, _vp, _u32]),
+        "clsph_nn_forward": (_i32, [_vp, _vp, _vp]),
+        "clsph_nn_infer": (_i32, [_vp, _vp]),
+        "clsph_nn_profile": (_i32, [_vp, _vp, _u32, _vp, _vp]),
+        "clsph_nn_read_graph": (_i32, [_vp, _vp, _vp, _u32, _u32]),
+        "clsph_nn_read_output": (_i32, [_vp, _vp, _u32]),
+        "clsph_nn_set_target": (_i32, [_vp, _vp, _u32]),
+        "clsph_nn_read_target": (_i32, [_vp, _vp, _u32]),
+        "clsph_nn_target_from": (_i32, [_vp, _vp, _vp, ctypes.c_float]),
+        "clsph_nn_backward": (_i32, [_vp, _vp]),
+        "clsph_nn_read_grad": (_i32, [_vp, _vp, _u32]),
+        "clsph_nn_adam": (_i32, [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                 ctypes.c_float]),
+        "clsph_learned_step": (_i32, [_vp, _vp, _u32]),
+        "clsph_nn_compare_density": (_i32, [_vp, _vp, _vp, _vp, _vp]),
+        "clsph_state_copy": (_i32, [_vp, _vp]),
+        "clsph_track_ids": (_i32, [_vp, _i32]),
+        "clsph_read_ids": (_i32, [_vp, _vp, _u32]),
+        "clsph_nn_gemm": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _u32, _u32, _u32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -295,6 +318,140 @@ class State:
         ms = np.zeros(1, dtype=np.float32)
         self.ctx.check(lib.clsph_time_stage(self.h, STAGES[name], reps, _ptr(ms)))
         return float(ms[0])
+
+    def copy_from(self, other):
+        """Take the particles (and per-particle fields) of another state."""
+        self.ctx.check(lib.clsph_state_copy(self.h, other.h))
+        self.n = other.n
+
+    def track_ids(self, enable=True):
+        """Reset identities to the current canonical index and keep them
+        through substeps (False stops tracking)."""
+        self.ctx.check(lib.clsph_track_ids(self.h, 1 if enable else 0))
+
+    def ids(self):
+        out = np.zeros(self.n, dtype=np.uint32)
+        self.ctx.check(lib.clsph_read_ids(self.h, _ptr(out), self.n))
+        return out
+
+    def learned_step(self, net, nsteps=1):
+        """Substeps with the network in place of the forces stage."""
+        self.ctx.check(lib.clsph_learned_step(self.h, net.h, nsteps))
+
+
+def nn_gemm(ctx, a_bits, b_bits, c, m, n, k, trans_a=False, trans_b=False, accumulate=False,
+            bias=None):
+    """The network's bf16 GEMM on host arrays (bf16 as uint16 bit patterns);
+    returns the float32 result (c updated in place)."""
+    a_bits = np.ascontiguousarray(a_bits, dtype=np.uint16)
+    b_bits = np.ascontiguousarray(b_bits, dtype=np.uint16)
+    c = np.ascontiguousarray(c, dtype=np.float32)
+    bias = None if bias is None else np.ascontiguousarray(bias, dtype=np.float32)
+    ctx.check(lib.clsph_nn_gemm(ctx.h, int(trans_a), int(trans_b), int(accumulate), _ptr(a_bits),
+                                _ptr(b_bits), _ptr(c), _ptr(bias), m, n, k))
+    return c
+
+
+class Network:
+    """The learned force surrogate's device workspace and parameters."""
+
+    def __init__(self, ctx, n_cap, e_cap=0):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(lib.clsph_nn_create(ctx.h, n_cap, e_cap, ctypes.byref(h)))
+        self.h = h
+        self.count = int(lib.clsph_nn_param_count())
+        self.n = 0
+        self.e = 0
+
+    def close(self):
+        if self.h:
+            lib.clsph_nn_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, flat):
+        flat = np.ascontiguousarray(flat, dtype=np.float32)
+        assert flat.shape == (self.count,)
+        self.ctx.check(lib.clsph_nn_set_params(self.h, _ptr(flat), self.count))
+
+    def params(self):
+        out = np.zeros(self.count, dtype=np.float32)
+        self.ctx.check(lib.clsph_nn_get_params(self.h, _ptr(out), self.count))
+        return out
+
+    def forward(self, state):
+        """Evaluate on a state whose last stage was density; returns the
+        edge count."""
+        e = np.zeros(1, dtype=np.uint32)
+        self.ctx.check(lib.clsph_nn_forward(self.h, state.h, _ptr(e)))
+        self.n, self.e = state.n, int(e[0])
+        return self.e
+
+    def infer(self, state):
+        """Evaluate with the fused inference kernels (no gradients)."""
+        self.ctx.check(lib.clsph_nn_infer(self.h, state.h))
+        self.n = state.n
+
+    def profile(self, state, reps=5):
+        """Average ms of (edge round 1, node round 1, edge round 2, node
+        round 2) over `reps` fused evaluations, and the edges per pass."""
+        ms = np.zeros(4, dtype=np.float32)
+        e = np.zeros(1, dtype=np.uint64)
+        self.ctx.check(lib.clsph_nn_profile(self.h, state.h, reps, _ptr(ms), _ptr(e)))
+        self.n = state.n
+        return [float(v) for v in ms], int(e[0])
+
+    def graph(self):
+        off = np.zeros(self.n + 1, dtype=np.uint32)
+        src = np.zeros(self.e, dtype=np.uint32)
+        self.ctx.check(lib.clsph_nn_read_graph(self.h, _ptr(off), _ptr(src), self.n, self.e))
+        return off, src
+
+    def output(self):
+        out = np.zeros((self.n, 32), dtype=np.float32)
+        self.ctx.check(lib.clsph_nn_read_output(self.h, _ptr(out), self.n))
+        return out
+
+    def set_target(self, target):
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        assert t.shape == (self.n, 3)
+        self.ctx.check(lib.clsph_nn_set_target(self.h, _ptr(t), self.n))
+
+    def target(self, n=None):
+        n = self.n if n is None else n
+        t = np.zeros((n, 3), dtype=np.float32)
+        self.ctx.check(lib.clsph_nn_read_target(self.h, _ptr(t), n))
+        return t
+
+    def target_from(self, learner, teacher, dt):
+        self.ctx.check(lib.clsph_nn_target_from(self.h, learner.h, teacher.h, ctypes.c_float(dt)))
+
+    def backward(self, read_loss=True):
+        loss = np.zeros(1, dtype=np.float32)
+        self.ctx.check(lib.clsph_nn_backward(self.h, _ptr(loss) if read_loss else None))
+        return float(loss[0]) if read_loss else None
+
+    def grad(self):
+        out = np.zeros(self.count, dtype=np.float32)
+        self.ctx.check(lib.clsph_nn_read_grad(self.h, _ptr(out), self.count))
+        return out
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.ctx.check(lib.clsph_nn_adam(self.h, ctypes.c_float(lr), ctypes.c_float(beta1),
+                                         ctypes.c_float(beta2), ctypes.c_float(eps)))
+
+    def compare_density(self, a, b):
+        """(max, mean) of |rho_a - rho_b| / rho0 over the same particles."""
+        mx = np.zeros(1, dtype=np.float32)
+        mn = np.zeros(1, dtype=np.float32)
+        self.ctx.check(lib.clsph_nn_compare_density(self.h, a.h, b.h, _ptr(mx), _ptr(mn)))
+        return float(mx[0]), float(mn[0])
 
 
 class Batch(State):

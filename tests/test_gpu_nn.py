@@ -1,0 +1,266 @@
+"""Learned force surrogate on the GPU against its CPU oracle (oracle/learned.py).
+
+  GEMM (all operand layouts, split K)        : 1e-5 of sum |a||b| per entry
+  neighbour graph                             : the density kernel's r < h set
+  forward vs the oracle with bf16 rounding    : output within 1 % of its scale
+  forward vs the exact FP64 network           : bf16 error reported, <= 5 %
+  gradients vs the FP64 oracle's backward     : per tensor, relative L2 <= 5 %
+  training targets, identities, zero network : exact / closed form
+"""
+import numpy as np
+import pytest
+
+from libclsph_amd import native, scenes
+from oracle import learned as L
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = native.Context(0)
+    print("device:", c.device_name(), flush=True)
+    yield c
+    c.close()
+
+
+def bits(x):
+    return L.bf16_bits(x)
+
+
+@pytest.mark.parametrize("ta,tb,acc,m,n,k", [
+    (False, False, False, 200, 136, 72),
+    (False, True, False, 200, 136, 72),
+    (True, False, False, 200, 136, 72),
+    (True, True, False, 200, 136, 72),
+    (False, False, True, 64, 32, 16),
+    (True, False, True, 128, 128, 9000),   # split over K, like a weight gradient
+    (True, False, True, 16, 128, 5000),
+])
+def test_gemm(ctx, ta, tb, acc, m, n, k):
+    print(f"gemm ta={ta} tb={tb} acc={acc} {m}x{n}x{k}", flush=True)
+    rng = np.random.default_rng(m * 7 + n + k)
+    a = rng.normal(size=(k, m) if ta else (m, k)).astype(np.float32)
+    b = rng.normal(size=(n, k) if tb else (k, n)).astype(np.float32)
+    c0 = rng.normal(size=(m, n)).astype(np.float32) if acc else np.zeros((m, n), np.float32)
+    bias = None if acc else rng.normal(size=n).astype(np.float32)
+    ab, bb = bits(a), bits(b)
+    got = native.nn_gemm(ctx, ab, bb, c0.copy(), m, n, k, ta, tb, acc, bias)
+    A = L.bits_to_float(ab).reshape(a.shape)
+    B = L.bits_to_float(bb).reshape(b.shape)
+    A = A.T if ta else A
+    B = B.T if tb else B
+    ref = A @ B + (c0 if acc else bias[None, :])
+    bound = np.abs(A) @ np.abs(B) + 1.0
+    assert np.all(np.abs(got - ref) <= 1e-5 * bound), np.abs(got - ref).max()
+
+
+def settled_state(ctx, n=3000, steps=30, seed=5):
+    sc = scenes.dam_break(n, seed=seed)
+    st = native.State(ctx, sc.n)
+    st.set_params(sc.params, sc.terms)
+    st.scene(sc.normals, sc.vertices, sc.indices)
+    st.upload(sc.particles)
+    st.step(steps)
+    for name in ("grid", "hash", "sort", "cell_table", "density"):
+        st.stage(name)
+    return sc, st
+
+
+def oracle_inputs(sc, st, net):
+    s = st.download_sorted()
+    off, src = net.graph()
+    deg = np.diff(off.astype(np.int64))
+    dst = np.repeat(np.arange(st.n), deg)
+    p = sc.params
+    return dict(x=s["position"][:, :3], v=s["velocity"][:, :3], rho=s["density"],
+                prs=s["pressure"], src=src.astype(np.int64), dst=dst, deg=deg,
+                h=float(p["h"]), rho0=float(p["fluid_density"]), K=float(p["K"]))
+
+
+def test_graph_is_the_density_neighbourhood(ctx):
+    sc, st = settled_state(ctx)
+    net = native.Network(ctx, st.n)
+    net.set_params(np.zeros(net.count, np.float32))
+    e = net.forward(st)
+    off, src = net.graph()
+    s = st.download_sorted()
+    x = s["position"][:, :3].astype(np.float32)
+    h2 = np.float32(sc.params["h"]) ** 2
+    assert off[-1] == e and e > 0
+    for i in range(0, st.n, 7):
+        d = x[i] - x
+        d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        want = set(np.nonzero(d2 < h2)[0]) - {i}
+        got = set(src[off[i]:off[i + 1]].tolist())
+        fuzzy = set(np.nonzero(np.abs(d2 - h2) <= 1e-5 * h2)[0])
+        assert (want ^ got) <= fuzzy, i
+        assert len(got) == off[i + 1] - off[i]
+
+
+def test_forward_matches_oracle(ctx):
+    sc, st = settled_state(ctx)
+    net = native.Network(ctx, st.n)
+    flat = L.random_params(3)
+    net.set_params(flat.astype(np.float32))
+    e = net.forward(st)
+    print(f"n={st.n} edges={e}", flush=True)
+    gpu = net.output().astype(np.float64)
+    args = oracle_inputs(sc, st, net)
+    # the GPU holds float32 parameters and rounds matrix operands to bf16
+    ref = L.forward(flat.astype(np.float32).astype(np.float64), bf16_round=True, **args)
+    exact = L.forward(flat.astype(np.float32).astype(np.float64), bf16_round=False, **args)
+    scale = np.abs(ref[:, :3]).max()
+    err = np.abs(gpu[:, :3] - ref[:, :3])
+    rms = np.sqrt((err ** 2).mean()) / np.sqrt((ref[:, :3] ** 2).mean())
+    ex = np.sqrt(((gpu[:, :3] - exact[:, :3]) ** 2).mean()) / np.sqrt((exact[:, :3] ** 2).mean())
+    print(f"max|gpu-oracle_bf16|/scale={err.max() / scale:.2e} rel.rms={rms:.2e} "
+          f"rel.rms vs fp64={ex:.2e}", flush=True)
+    assert err.max() <= 1e-2 * scale
+    assert rms <= 2e-3
+    assert ex <= 5e-2
+    assert np.all(gpu[:, 3:] == 0.0)
+
+
+def test_gradients_match_oracle(ctx):
+    sc, st = settled_state(ctx, n=2000)
+    net = native.Network(ctx, st.n)
+    flat = L.random_params(4).astype(np.float32).astype(np.float64)
+    net.set_params(flat.astype(np.float32))
+    net.forward(st)
+    rng = np.random.default_rng(0)
+    target = rng.normal(0, 0.5, size=(st.n, 3)).astype(np.float32)
+    net.set_target(target)
+    loss = net.backward()
+    g = net.grad().astype(np.float64)
+    args = oracle_inputs(sc, st, net)
+    out, c = L.forward(flat, bf16_round=True, keep=True, **args)
+    ref_loss = L.loss(out, target.astype(np.float64))
+    gref = L.backward(flat, c, target.astype(np.float64), args["src"], args["dst"], args["deg"])
+    print(f"loss gpu={loss:.6e} oracle={ref_loss:.6e}", flush=True)
+    assert abs(loss - ref_loss) <= 1e-3 * ref_loss
+    G, R = L.unpack(g), L.unpack(gref)
+    worst = 0.0
+    for name, _ in L.LAYOUT:
+        gn, rn = G[name].ravel(), R[name].ravel()
+        nr = np.linalg.norm(rn)
+        if nr == 0.0:
+            assert np.all(gn == 0.0), name
+            continue
+        rel = np.linalg.norm(gn - rn) / nr
+        worst = max(worst, rel)
+        print(f"  {name:6s} |g|={nr:.3e} rel.err={rel:.2e}", flush=True)
+        assert rel <= 5e-2, name
+    print(f"worst relative gradient error {worst:.2e}", flush=True)
+
+
+def test_target_and_identities(ctx):
+    sc, st = settled_state(ctx, n=2500, steps=10)
+    teacher = native.State(ctx, st.n)
+    teacher.set_params(sc.params, sc.terms)
+    teacher.scene(sc.normals, sc.vertices, sc.indices)
+    teacher.copy_from(st)
+    teacher.track_ids(True)
+    teacher.step(4)
+    ids = teacher.ids()
+    assert np.array_equal(np.sort(ids), np.arange(st.n))
+    net = native.Network(ctx, st.n)
+    dt = 4.0 * float(sc.params["time_delta"])
+    net.n = st.n
+    net.target_from(st, teacher, dt)
+    got = net.target(st.n).astype(np.float64)
+    # host restatement: learner's sorted i <- canonical perm[i]; the teacher
+    # holds that particle at loc[perm[i]]
+    _, perm = st.sorted()
+    canon = st.download()
+    t_parts = teacher.download()
+    loc = np.empty(st.n, np.int64)
+    loc[ids] = np.arange(st.n)
+    g = sc.params["constant_acceleration"][:3].astype(np.float64)
+    vs = canon["intermediate_velocity"][:, :3].astype(np.float32)
+    vt = t_parts["intermediate_velocity"][:, :3].astype(np.float32)
+    want = ((vt[loc[perm]] - vs[perm]).astype(np.float32) * np.float32(1.0 / dt)
+            - g.astype(np.float32)) / np.float32(10.0)
+    assert np.allclose(got, want, rtol=1e-5, atol=1e-6)
+    # a particle moves continuously: over 4 substeps it stays within 4 dt vmax
+    x0 = canon["position"][:, :3]
+    x1 = t_parts["position"][:, :3][loc]
+    assert np.abs(x1 - x0).max() < 4 * float(sc.params["time_delta"]) * 5.0
+
+
+def test_zero_network_is_free_fall(ctx):
+    sc = scenes.dam_break(2048, seed=2)
+    parts = sc.particles.copy()
+    parts["position"][:, 1] += np.float32(0.05)  # lift off the floor
+    st = native.State(ctx, sc.n)
+    st.set_params(sc.params, sc.terms)
+    st.scene(sc.normals, sc.vertices, sc.indices)
+    st.upload(parts)
+    net = native.Network(ctx, st.n)
+    net.set_params(np.zeros(net.count, np.float32))
+    st.learned_step(net, 1)
+    out = st.download()
+    dt = np.float32(sc.params["time_delta"])
+    g = sc.params["constant_acceleration"][:3]
+    vh = out["intermediate_velocity"][:, :3]
+    assert np.allclose(vh, np.broadcast_to(g * dt, vh.shape), rtol=1e-6, atol=1e-9)
+    # positions moved by (g dt) dt: the set of heights drops by that amount
+    y0 = np.sort(parts["position"][:, 1])
+    y1 = np.sort(out["position"][:, 1])
+    assert np.allclose(y1, y0 + g[1] * dt * dt, atol=1e-6)
+
+
+def test_compare_density_of_a_copy_is_zero(ctx):
+    sc, st = settled_state(ctx, n=2000, steps=5)
+    other = native.State(ctx, st.n)
+    other.set_params(sc.params, sc.terms)
+    other.scene(sc.normals, sc.vertices, sc.indices)
+    other.copy_from(st)
+    st.track_ids(True)
+    other.track_ids(True)
+    for s in (st, other):
+        s.step(2)
+        for name in ("grid", "hash", "sort", "cell_table", "density"):
+            s.stage(name)
+    net = native.Network(ctx, st.n)
+    mx, mean = net.compare_density(st, other)
+    assert mx == 0.0 and mean == 0.0
+
+
+def test_training_reduces_loss(ctx):
+    sc, st = settled_state(ctx, n=3000, steps=20)
+    teacher = native.State(ctx, st.n)
+    teacher.set_params(sc.params, sc.terms)
+    teacher.scene(sc.normals, sc.vertices, sc.indices)
+    teacher.copy_from(st)
+    teacher.track_ids(True)
+    teacher.step(4)
+    net = native.Network(ctx, st.n)
+    net.set_params(L.init_params(0).astype(np.float32))
+    net.forward(st)
+    net.target_from(st, teacher, 4.0 * float(sc.params["time_delta"]))
+    losses = []
+    for _ in range(60):
+        net.forward(st)
+        losses.append(net.backward())
+        net.adam(1e-3)
+    print("loss", losses[0], "->", losses[-1], flush=True)
+    assert np.all(np.isfinite(losses))
+    assert losses[-1] < 0.5 * losses[0]
+
+
+def test_fused_inference_matches_training_forward(ctx):
+    sc, st = settled_state(ctx, n=4000, steps=40, seed=9)
+    net = native.Network(ctx, st.n)
+    net.set_params(L.random_params(11).astype(np.float32))
+    net.forward(st)
+    ref = net.output().astype(np.float64)
+    net.infer(st)
+    got = net.output().astype(np.float64)
+    scale = np.abs(ref[:, :3]).max()
+    err = np.abs(got[:, :3] - ref[:, :3])
+    rms = np.sqrt((err ** 2).mean()) / np.sqrt((ref[:, :3] ** 2).mean())
+    print(f"fused vs training forward: max {err.max() / scale:.2e} of scale, rel.rms {rms:.2e}",
+          flush=True)
+    assert err.max() <= 1e-2 * scale
+    assert rms <= 1e-3
