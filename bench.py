@@ -90,6 +90,11 @@ def main():
     ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--track-ids", action="store_true",
+                    help="also track particle identities each substep; its copy kernel "
+                         "(k_copy_u32, 4 n bytes read and written) calibrates HBM counters")
+    ap.add_argument("--learned-macro-steps", type=int, default=100)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -123,6 +128,8 @@ def main():
     st.scene(scene.normals, scene.vertices, scene.indices)
     st.upload(scene.particles)
     print(f"# device {ctx.device_name()}; {n} particles", file=sys.stderr, flush=True)
+    if args.track_ids:
+        st.track_ids(True)
 
     if args.warmup:
         st.step(args.warmup)
@@ -151,6 +158,10 @@ def main():
     final = st.download()
     finite = bool(np.isfinite(final["position"][:, :3]).all())
     capped = st.bounce_capped()
+
+    learned = None
+    if args.workload == "dam_break" and not args.no_learned:
+        learned = bench_learned(ctx, st, scene, ms_per_step, args.learned_macro_steps)
 
     cfg.update({
         "dt": float(scene.params["time_delta"]),
@@ -184,6 +195,8 @@ def main():
             "traffic": None,
         },
     }
+    if learned is not None:
+        out["learned"] = learned
     if not args.no_cpu_baseline:
         if args.workload == "dam_break":
             out["cpu_baseline"] = cpu_baseline(scene, f"the {n}-particle dam break")
@@ -194,6 +207,81 @@ def main():
                 sub, f"the bottom 65,536 particles of the {n}-particle column (every particle "
                      f"tested against all {len(scene.indices) // 3} triangles)")
     print(json.dumps(out), flush=True)
+
+
+# Learned surrogate: multiply-adds per evaluated edge and round in the fused
+# edge kernel (layer 1: 16 inputs -> 128; layer 2: 128 -> 128).
+NN_EDGE_FLOP = 2.0 * (16 * 128 + 128 * 128)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def bench_learned(ctx, plain, scene, plain_ms_per_step, macro_steps, judge_every=8, tol=0.01):
+    """The learned surrogate (trained weights shipped in-tree) against the
+    plain method from the same state: simulated seconds per wall second of
+    each, the judge's fallbacks, and the density error against the plain run
+    at the matched time (max and mean |rho - rho_plain| / rho0), beside the
+    literal max |rho / rho0 - 1| of both."""
+    from libclsph_amd import learned as lrn
+    if not os.path.exists(lrn.WEIGHTS):
+        return {"skipped": f"no trained weights at {lrn.WEIGHTS}"}
+    n = plain.n
+    dt = float(scene.params["time_delta"])
+    learner = lrn.make_state(ctx, scene, dt=lrn.MACRO * dt)
+    teacher = lrn.make_state(ctx, scene)
+    net = native.Network(ctx, n, e_cap=1024)
+    net.set_params(np.load(lrn.WEIGHTS))
+    plain.track_ids(True)
+    learner.track_ids(True)
+    teacher.track_ids(True)
+    learner.copy_from(plain)
+    # kernel timing of the network on the current state
+    lrn.density_stages(learner)
+    ms, edges = net.profile(learner, reps=5)
+    r = lrn.run_learned(ctx, learner, teacher, net, macro_steps, dt, judge_every, tol)
+    plain.step(macro_steps * lrn.MACRO)
+    lrn.density_stages(plain)
+    lrn.density_stages(learner)
+    err_max, err_mean = net.compare_density(learner, plain)
+    rho0 = float(scene.params["fluid_density"])
+    lit = {}
+    for name, s in (("learned", learner), ("plain", plain)):
+        d = s.download_sorted()["density"].astype(np.float64)
+        lit[name] = float(np.abs(d / rho0 - 1.0).max())
+    # learned macro steps without the judge, for the network's own cost
+    learner.copy_from(plain)
+    ctx.sync()
+    t0 = time.perf_counter()
+    learner.learned_step(net, 20)
+    ctx.sync()
+    bare_ms = (time.perf_counter() - t0) / 20 * 1e3
+    plain_rate = dt / (plain_ms_per_step * 1e-3)
+    learned_rate = r.sim_s / r.wall_s
+    edge_ms = ms[0] + ms[2]
+    for s in (learner, teacher):
+        s.close()
+    plain.track_ids(False)
+    net.close()
+    return {
+        "sim_s_per_wall_s": {"plain": plain_rate, "learned": learned_rate,
+                             "ratio": learned_rate / plain_rate},
+        "macro_step_dt": lrn.MACRO * dt,
+        "macro_steps": r.macro_steps,
+        "judge_every": judge_every,
+        "judged": r.judged,
+        "fallbacks": r.fallbacks,
+        "judge_max_density_err": max(r.judge_err) if r.judge_err else None,
+        "density_err_vs_plain": {"max": err_max, "mean": err_mean,
+                                 "sim_time_s": r.sim_s},
+        "literal_max_abs_rho_over_rho0_minus_1": lit,
+        "ms_per_macro_step_unjudged": bare_ms,
+        "network_ms": {"edge_round1": ms[0], "node_round1": ms[1], "edge_round2": ms[2],
+                       "node_round2": ms[3]},
+        "edges": edges,
+        "edge_kernel_roofline": {
+            "bound": "mfma", "unit": "TFLOP/s", "peak": MFMA_BF16_PEAK_TFLOPS,
+            "achieved": 2 * edges * NN_EDGE_FLOP / (edge_ms * 1e-3) / 1e12,
+            "frac": 2 * edges * NN_EDGE_FLOP / (edge_ms * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS},
+    }
 
 
 def bench_batched(args):

@@ -57,9 +57,21 @@ def time_config(state, sizes, steps):
     return state.time_steps(steps) / steps
 
 
-def tune_state(state, steps=20, sweeps=2, log=print):
+def ab_test(state, best, trial, steps):
+    """Times trial, best, trial back to back on the (evolving) state and
+    returns (best ms/substep of the trial, ms/substep of the incumbent)."""
+    t1 = time_config(state, trial, steps)
+    t0 = time_config(state, best, steps)
+    t2 = time_config(state, trial, steps)
+    return min(t1, t2), t0
+
+
+def tune_state(state, steps=20, sweeps=2, log=print, margin=0.01):
+    """Coordinate descent over the tuned stages' work-group sizes. A trial
+    replaces the incumbent only when it beats it by `margin` in a trial /
+    incumbent / trial sequence, so that drift of the evolving scene and
+    timing noise do not decide."""
     sizes = list(state.block_sizes())
-    best = time_config(state, sizes, steps)
     for _ in range(sweeps):
         for name, stage in TUNED.items():
             for c in CANDIDATES:
@@ -67,12 +79,12 @@ def tune_state(state, steps=20, sweeps=2, log=print):
                     continue
                 trial = list(sizes)
                 trial[stage] = c
-                t = time_config(state, trial, steps)
-                if t < best * 0.995:
-                    best, sizes = t, trial
-                    log(f"  {name} -> {c}: {t:.4f} ms/substep")
+                t, t_best = ab_test(state, sizes, trial, steps)
+                if t < t_best * (1.0 - margin):
+                    sizes = trial
+                    log(f"  {name} -> {c}: {t:.4f} vs {t_best:.4f} ms/substep")
     state.set_block_sizes(sizes)
-    return sizes, best
+    return sizes
 
 
 def make_batch(ctx, scenes):
@@ -96,8 +108,8 @@ def tune_buckets(buckets, scenes_per_bucket=16, steps=20, log=print):
         batch = make_batch(ctx, [scenes.dam_break(n, seed=7 + i, spacing=spacing)
                                  for i in range(k)])
         batch.step(20)
-        base = time_config(batch, list(DEFAULT), steps)
-        sizes, best = tune_state(batch, steps=steps, log=log)
+        sizes = tune_state(batch, steps=steps, log=log)
+        best, base = ab_test(batch, list(DEFAULT), sizes, steps)
         log(f"bucket 2^{bk} ({k} scenes): default {base:.4f} ms, tuned {best:.4f} ms, {sizes}")
         out[bk] = {"sizes": sizes, "ms_default": base, "ms_tuned": best, "scenes": k}
         batch.close()

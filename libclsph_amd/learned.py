@@ -211,7 +211,7 @@ def main():
     ap.add_argument("--init", default=None, help="start from these weights")
     a = ap.parse_args()
     ctx = native.Context(0)
-    n_cap = 100 * 200 * 6  # largest slab variant: 100 x 200 x 6 sites
+    n_cap = 125000  # largest slab variant: 101 x 202 x 6 sites
     net = native.Network(ctx, n_cap, e_cap=96 * n_cap)
     net.set_params(np.load(a.init) if a.init else init_params(a.seed))
     losses = train(net, ctx, trajectories=a.trajectories, steps=a.steps, gap=a.gap,

@@ -218,9 +218,11 @@ class State:
         self.n = 0
 
     def close(self):
-        if self.h:
+        # freeing needs the context's stream: after the context is closed the
+        # buffers are left to the end of the process
+        if self.h and self.ctx.h:
             lib.clsph_state_free(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -365,9 +367,9 @@ class Network:
         self.e = 0
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib.clsph_nn_free(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -1,0 +1,181 @@
+"""The trained surrogate (libclsph_amd/weights) against the method without it.
+
+On a dam-break variant the training never saw, from one common state:
+  (a) the plain method at dt                (the reference trajectory)
+  (b) the plain method at 4 dt              (the method without the network)
+  (c) the learned method at 4 dt            (the network in place of forces)
+Densities of the same particles are compared at the matched time. The
+learned run must stay finite with its density error against (a) inside the
+judge's bound, and the test prints what it gains over (b). A separate case
+compares both GPU runs with an FP64 trajectory of the CPU oracle.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from libclsph_amd import learned, native, scenes
+from sorted_view import internal_sorted
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not os.path.exists(learned.WEIGHTS),
+                                 reason="no trained surrogate weights in the tree")]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = native.Context(0)
+    print("device:", c.device_name(), flush=True)
+    yield c
+    c.close()
+
+
+def start_state(ctx, sc, warm):
+    st = learned.make_state(ctx, sc)
+    st.upload(sc.particles)
+    st.step(warm)
+    st.track_ids(True)
+    return st
+
+
+def density_by_id(st):
+    """Densities indexed by particle identity (after the density stages)."""
+    s, perm = internal_sorted(st)
+    ids = st.ids()
+    out = np.empty(st.n)
+    out[ids[perm]] = s["density"]
+    return out
+
+
+def test_learned_gain_over_plain_at_large_step(ctx):
+    sc = learned.slab_variant(100003)
+    dt = float(sc.params["time_delta"])
+    rho0 = float(sc.params["fluid_density"])
+    macro = 25
+    base = start_state(ctx, sc, warm=300)
+    runs = {}
+    for name in ("plain_dt", "plain_4dt", "learned_4dt"):
+        big = name != "plain_dt"
+        st = learned.make_state(ctx, sc, dt=learned.MACRO * dt if big else None)
+        st.track_ids(True)
+        base.track_ids(True)
+        st.copy_from(base)
+        runs[name] = st
+    net = native.Network(ctx, sc.n, e_cap=1024)
+    net.set_params(np.load(learned.WEIGHTS))
+    runs["plain_dt"].step(macro * learned.MACRO)
+    runs["plain_4dt"].step(macro)
+    runs["learned_4dt"].learned_step(net, macro)
+    rho = {}
+    for name, st in runs.items():
+        learned.density_stages(st)
+        rho[name] = density_by_id(st)
+        assert np.isfinite(rho[name]).all(), name
+    err = {k: np.abs(rho[k] - rho["plain_dt"]) / rho0 for k in ("plain_4dt", "learned_4dt")}
+    for k, e in err.items():
+        print(f"{k}: max |rho - rho_plain| / rho0 = {e.max():.3e}, mean {e.mean():.3e}",
+              flush=True)
+    mx, mean = net.compare_density(runs["learned_4dt"], runs["plain_dt"])
+    assert abs(mx - err["learned_4dt"].max()) <= 1e-6
+    # the learned run stays within the judge's bound of the plain run
+    assert err["learned_4dt"].max() <= 0.05
+    print(f"gain in max density error over plain at 4 dt: "
+          f"{err['plain_4dt'].max() / max(err['learned_4dt'].max(), 1e-12):.2f}x", flush=True)
+
+
+def test_judged_run_counts_fallbacks(ctx):
+    sc = learned.slab_variant(100019)
+    dt = float(sc.params["time_delta"])
+    base = start_state(ctx, sc, warm=200)
+    learner = learned.make_state(ctx, sc, dt=learned.MACRO * dt)
+    teacher = learned.make_state(ctx, sc)
+    learner.track_ids(True)
+    teacher.track_ids(True)
+    learner.copy_from(base)
+    net = native.Network(ctx, sc.n, e_cap=1024)
+    net.set_params(np.load(learned.WEIGHTS))
+    r = learned.run_learned(ctx, learner, teacher, net, 16, dt, judge_every=2, tol=0.01)
+    print(f"judged {r.judged}, fallbacks {r.fallbacks}, judge errors "
+          f"{np.round(r.judge_err, 5).tolist()}", flush=True)
+    assert r.judged == 8 and 0 <= r.fallbacks <= r.judged
+    assert all(np.isfinite(r.judge_err))
+    # a tolerance of 0 forces a fallback at every judged step: the result is
+    # then the plain trajectory at the judged steps
+    learner.copy_from(base)
+    r0 = learned.run_learned(ctx, learner, teacher, net, 2, dt, judge_every=1, tol=0.0)
+    assert r0.fallbacks == 2
+    plain = learned.make_state(ctx, sc)
+    plain.track_ids(True)
+    base.track_ids(True)
+    plain.copy_from(base)
+    learner.track_ids(True)
+    learner.copy_from(base)
+    learned.run_learned(ctx, learner, teacher, net, 2, dt, judge_every=1, tol=0.0)
+    plain.step(2 * learned.MACRO)
+    learned.density_stages(plain)
+    learned.density_stages(learner)
+    mx, _ = net.compare_density(learner, plain)
+    assert mx == 0.0
+
+
+def fp64_trajectory(sc, parts, steps):
+    """FP64 oracle trajectory: grid and sort in FP32 (the reference's cell
+    ids) on positions rounded from FP64, density, forces and advection in
+    FP64 (oracle *_f64). Particle identities ride in position[3]. Returns
+    the densities at the end, indexed by particle identity."""
+    p = sc.params.copy()
+    p32 = parts.copy()
+    p32["position"][:, 3] = np.arange(len(parts), dtype=np.float32)
+    p64 = oracle.to_f64(p32)
+    by_id = p64.copy()
+
+    def sorted_state():
+        cur = np.zeros_like(p32)
+        for f in ("position", "velocity", "intermediate_velocity", "acceleration"):
+            cur[f] = by_id[f].astype(np.float32)
+        cur["position"][:, 3] = np.arange(len(parts), dtype=np.float32)
+        oracle.grid_params(cur, p)
+        s32 = oracle.sort_particles(oracle.locate_in_grid(cur, p))
+        ids = s32["position"][:, 3].astype(np.int64)
+        table = oracle.cell_table(s32, int(p["grid_cell_count"]))
+        s64 = by_id[ids].copy()
+        return ids, table, s64
+
+    for _ in range(steps):
+        ids, table, s64 = sorted_state()
+        d = oracle.density_pressure_f64(s64, p, sc.terms, table)
+        f = oracle.forces_f64(d, p, sc.terms, table)
+        a, _ = oracle.advection_collision_f64(f, p, sc.normals, sc.vertices, sc.indices)
+        by_id[ids] = a
+    ids, table, s64 = sorted_state()
+    d = oracle.density_pressure_f64(s64, p, sc.terms, table)
+    rho = np.empty(len(parts))
+    rho[ids] = d["density"]
+    return rho
+
+
+def test_error_against_fp64_oracle(ctx):
+    sc = scenes.dam_break(4096, seed=77, spacing=learned.SPACING_1M)
+    dt = float(sc.params["time_delta"])
+    rho0 = float(sc.params["fluid_density"])
+    macro = 10
+    plain = learned.make_state(ctx, sc)
+    plain.upload(sc.particles)
+    plain.track_ids(True)
+    learner = learned.make_state(ctx, sc, dt=learned.MACRO * dt)
+    learner.track_ids(True)
+    learner.copy_from(plain)
+    net = native.Network(ctx, sc.n, e_cap=1024)
+    net.set_params(np.load(learned.WEIGHTS))
+    plain.step(macro * learned.MACRO)
+    learner.learned_step(net, macro)
+    ref = fp64_trajectory(sc, sc.particles, macro * learned.MACRO)
+    out = {}
+    for name, st in (("plain_f32", plain), ("learned", learner)):
+        learned.density_stages(st)
+        e = np.abs(density_by_id(st) - ref) / rho0
+        out[name] = e
+        print(f"{name} vs FP64 oracle: max {e.max():.3e}, mean {e.mean():.3e}", flush=True)
+    assert out["plain_f32"].max() <= 1e-4
+    assert np.isfinite(out["learned"]).all()

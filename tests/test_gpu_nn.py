@@ -12,6 +12,7 @@ import pytest
 
 from libclsph_amd import native, scenes
 from oracle import learned as L
+from sorted_view import internal_sorted
 
 pytestmark = pytest.mark.gpu
 
@@ -68,7 +69,7 @@ def settled_state(ctx, n=3000, steps=30, seed=5):
 
 
 def oracle_inputs(sc, st, net):
-    s = st.download_sorted()
+    s, _ = internal_sorted(st)
     off, src = net.graph()
     deg = np.diff(off.astype(np.int64))
     dst = np.repeat(np.arange(st.n), deg)
@@ -84,7 +85,7 @@ def test_graph_is_the_density_neighbourhood(ctx):
     net.set_params(np.zeros(net.count, np.float32))
     e = net.forward(st)
     off, src = net.graph()
-    s = st.download_sorted()
+    s, _ = internal_sorted(st)
     x = s["position"][:, :3].astype(np.float32)
     h2 = np.float32(sc.params["h"]) ** 2
     assert off[-1] == e and e > 0
