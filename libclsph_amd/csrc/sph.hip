@@ -75,10 +75,63 @@ __device__ inline void for_each_candidate(const uint32_t hc[3], uint32_t base,
     }
 }
 
+// Half-cell coordinates and the particle's fractional position inside its
+// half-cell (in units of the half-cell side h), with the hash's arithmetic.
+__device__ inline void half_cell_frac(float4 x, const DevParams* __restrict__ dp, uint32_t hc[3],
+                                      float fr[3]) {
+#pragma clang fp contract(off)
+  float L = dp->p.h * 2;
+  float u[3] = {2.f * ((x.x - dp->p.min_point.s[0]) / L),
+                2.f * ((x.y - dp->p.min_point.s[1]) / L),
+                2.f * ((x.z - dp->p.min_point.s[2]) / L)};
+  for (int a = 0; a < 3; ++a) {
+    hc[a] = (uint32_t)u[a];
+    fr[a] = u[a] - (float)hc[a];
+  }
+}
+
+// Squared reach, in half-cell units, beyond which a half-cell cannot hold a
+// particle within h: (1.01)^2. The 1 % margin exceeds the FP32 rounding of
+// the scaled coordinates (a few ulp of at most 2048 half-cells, < 1e-3).
+constexpr float kReach2 = 1.0201f;
+
+// Visits the particle ranges of the half-cells around hc that a sphere of
+// radius h about the particle can reach: a (dy, dz) row is skipped when its
+// nearest face is out of reach, and its x-range keeps the neighbours on
+// either side only when they are in reach. Within a row the half-cells
+// 2k, 2k+1 share a cell and are Morton-adjacent; each maximal such piece
+// of the x-range is one run (at most two per row).
+template <typename R>
+__device__ inline void for_each_run(const uint32_t hc[3], const float fr[3], uint32_t base,
+                                    const uint32_t* __restrict__ ht, uint32_t Fc, uint32_t n,
+                                    R&& run) {
+  const float lo_x = fr[0] * fr[0], hi_x = (1.f - fr[0]) * (1.f - fr[0]);
+  for (int dz = -1; dz <= 1; ++dz) {
+    const float az = dz < 0 ? fr[2] : (dz > 0 ? 1.f - fr[2] : 0.f);
+    for (int dy = -1; dy <= 1; ++dy) {
+      const float ay = dy < 0 ? fr[1] : (dy > 0 ? 1.f - fr[1] : 0.f);
+      const float ryz = az * az + ay * ay;
+      if (ryz >= kReach2) continue;
+      const uint32_t y = hc[1] + (uint32_t)dy, z = hc[2] + (uint32_t)dz;
+      const uint32_t xa = lo_x + ryz < kReach2 ? hc[0] - 1u : hc[0];
+      const uint32_t xb = hi_x + ryz < kReach2 ? hc[0] + 1u : hc[0];
+      uint32_t x = xa;
+      for (int piece = 0; piece < 2 && x <= xb; ++piece) {
+        const uint32_t xe = (x | 1u) < xb ? (x | 1u) : xb;
+        uint32_t s, e;
+        half_range(base + half_key(x, y, z), xe - x + 1u, ht, Fc, n, s, e);
+        run(s, e);
+        x = xe + 1u;
+      }
+    }
+  }
+}
+
 // Besides density and pressure, records the particle's neighbours (r < h,
 // self excluded) in visiting order, four to a uint4, into
 // nbr[q * n + i] (q < kMaxNeighbours / 4), the count into nbr_count[i], and
 // the force inputs fa[i] = (x, p / rho^2) and fb[i] = (v, m / rho).
+// Candidates are loaded four at a time so that their latencies overlap.
 __global__ __launch_bounds__(kMaxBlock) void k_density(const float4* __restrict__ pos,
                                                        const float4* __restrict__ vel,
                                                        const uint32_t* __restrict__ table,
@@ -100,27 +153,34 @@ __global__ __launch_bounds__(kMaxBlock) void k_density(const float4* __restrict_
   const uint32_t F = usable_key_end(dp, bp);
   float4 xi = pos[i];
   uint32_t hc[3];
-  half_cell(xi, dp, hc);
+  float fr[3];
+  half_cell_frac(xi, dp, hc, fr);
   float acc = 0.f;
   uint32_t cnt = 0;
   uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;  // pending list entries
-  for_each_candidate(hc, dp->key_base, table, F, n, [&](uint32_t j) {
-    float4 xj = pos[j];
-    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-    float d2 = dx * dx + dy * dy + dz * dz;
-    if (d2 < h2) {
-      float t = h2 - d2;
-      acc += t * t * t;
-      if (j != i) {
-        if (cnt < kMaxNeighbours) {
-          uint32_t c = cnt & 3u;
-          p0 = c == 0 ? j : p0;
-          p1 = c == 1 ? j : p1;
-          p2 = c == 2 ? j : p2;
-          p3 = c == 3 ? j : p3;
-          if (c == 3) nbr[(size_t)(cnt >> 2) * n + i] = make_uint4(p0, p1, p2, p3);
+  for_each_run(hc, fr, dp->key_base, table, F, n, [&](uint32_t s, uint32_t e) {
+    for (uint32_t j0 = s; j0 < e; j0 += 4) {
+      float4 c[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) c[u] = pos[j0 + u < e ? j0 + u : e - 1];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t j = j0 + u;
+        float dx = xi.x - c[u].x, dy = xi.y - c[u].y, dz = xi.z - c[u].z;
+        float d2 = dx * dx + dy * dy + dz * dz;
+        if (j < e && d2 < h2) {
+          float t = h2 - d2;
+          acc += t * t * t;
+          const bool app = j != i;
+          const bool keep = app && cnt < kMaxNeighbours;
+          const uint32_t q = cnt & 3u;
+          p0 = keep && q == 0 ? j : p0;
+          p1 = keep && q == 1 ? j : p1;
+          p2 = keep && q == 2 ? j : p2;
+          p3 = keep && q == 3 ? j : p3;
+          if (keep && q == 3) nbr[(size_t)(cnt >> 2) * n + i] = make_uint4(p0, p1, p2, p3);
+          cnt += app ? 1u : 0u;
         }
-        ++cnt;
       }
     }
   });
