@@ -141,6 +141,12 @@ int clsph_get_block_sizes(const clsph_state* st, uint32_t sizes[7]);
 /* ---- readback of stage results (for checks and the save delegates) ---- */
 /* Device-computed grid fields of the current substep, written into *params. */
 int clsph_get_grid(clsph_state* st, clsph_simulation_parameters* params);
+/* The kernels' own particle order of the current substep (half-cell order:
+ * by cell id, then octant, stable): for internal position k, slot[k] is its
+ * position in the reference's sorted order (clsph_read_sorted,
+ * clsph_download_sorted_aos) and perm[k] its canonical index. The learned
+ * surrogate's graph, outputs and targets are indexed this way. */
+int clsph_read_order(clsph_state* st, uint32_t* slot, uint32_t* perm, uint32_t n);
 /* Cell ids in canonical (pre-sort) order after clsph_hash. */
 int clsph_read_keys(clsph_state* st, uint32_t* keys, uint32_t n);
 /* Sorted cell ids and the permutation (sorted slot -> canonical index). */

@@ -767,6 +767,20 @@ int clsph_get_grid(clsph_state* st, clsph_simulation_parameters* params) {
   return clsph_batch_get_grid(st, 0, params);
 }
 
+int clsph_read_order(clsph_state* st, uint32_t* slot, uint32_t* perm, uint32_t n) {
+  if (!st) return CLSPH_EINVAL;
+  if (n != st->scene_n[0]) return fail(st->ctx, CLSPH_EINVAL, "n must be the particle count");
+  uint32_t b0 = st->scene_begin[0];
+  int rc = CLSPH_OK;
+  if (slot) rc = read_u32(st, st->slot + b0, slot, n);
+  if (!rc && perm) rc = read_u32(st, st->val_b + b0, perm, n);
+  for (uint32_t i = 0; !rc && i < n; ++i) {
+    if (slot) slot[i] -= b0;
+    if (perm) perm[i] -= b0;
+  }
+  return rc;
+}
+
 int clsph_read_keys(clsph_state* st, uint32_t* keys, uint32_t n) {
   if (!st) return CLSPH_EINVAL;
   if (n > st->scene_n[0]) return fail(st->ctx, CLSPH_EINVAL, "n exceeds the particle count");

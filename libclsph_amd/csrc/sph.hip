@@ -221,17 +221,11 @@ __device__ inline void force_pair(ForceSums& f, const ForceConsts& k, float4 ai,
   float hr = k.h - r;
   // pressure (forces.cl:66-73): (p_j/rho_j^2 + p_i/rho_i^2) m grad W_spiky
   float ps = (aj.w + ai.w) * k.m;
-  if (r < 1e-7f) {
-    float g = ps * k.spiky0;
-    f.pt.x += g;
-    f.pt.y += g;
-    f.pt.z += g;
-  } else {
-    float g = ps * k.c_spiky * hr * hr * __builtin_amdgcn_rcpf(r);
-    f.pt.x += g * dx;
-    f.pt.y += g * dy;
-    f.pt.z += g * dz;
-  }
+  const bool coincident = r < 1e-7f;
+  float g = coincident ? ps * k.spiky0 : ps * k.c_spiky * hr * hr * __builtin_amdgcn_rcpf(r);
+  f.pt.x += g * (coincident ? 1.f : dx);
+  f.pt.y += g * (coincident ? 1.f : dy);
+  f.pt.z += g * (coincident ? 1.f : dz);
   // viscosity (forces.cl:75-80): (v_j - v_i) (m/rho_j) lap W_visc
   float vw = bj.w * k.c_visc * hr;
   f.vt.x += (bj.x - bi.x) * vw;
@@ -281,16 +275,23 @@ __global__ __launch_bounds__(kMaxBlock) void k_forces(const float4* __restrict__
   fs.lap = bi.w * k.c_lap * k.h2 * (3.f * k.h2);
   uint32_t cnt = nbr_count[i];
   if (cnt <= kMaxNeighbours) {
-    uint32_t q = 0;
-    for (; q + 4 <= cnt; q += 4) {
-      uint4 js = nbr[(size_t)(q >> 2) * n + i];
+    // groups of four neighbours; the next group's indices load while the
+    // current group's particles are gathered
+    const uint32_t groups = cnt >> 2;
+    uint4 js = make_uint4(0u, 0u, 0u, 0u);
+    if (groups) js = nbr[i];
+    for (uint32_t gq = 0; gq < groups; ++gq) {
+      uint4 jn = js;
+      if (gq + 1 < groups) jn = nbr[(size_t)(gq + 1) * n + i];
       float4 a0 = fa[js.x], b0 = fb[js.x], a1 = fa[js.y], b1 = fb[js.y];
       float4 a2 = fa[js.z], b2 = fb[js.z], a3 = fa[js.w], b3 = fb[js.w];
       force_pair(fs, k, ai, bi, a0, b0);
       force_pair(fs, k, ai, bi, a1, b1);
       force_pair(fs, k, ai, bi, a2, b2);
       force_pair(fs, k, ai, bi, a3, b3);
+      js = jn;
     }
+    const uint32_t q = groups << 2;
     if (q < cnt) {
       uint4 js = nbr[(size_t)(q >> 2) * n + i];
       uint32_t r = cnt - q;

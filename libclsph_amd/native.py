@@ -62,6 +62,7 @@ def _load():
         "clsph_read_keys": (_i32, [_vp, _vp, _u32]),
         "clsph_read_sorted": (_i32, [_vp, _vp, _vp, _u32]),
         "clsph_read_cell_table": (_i32, [_vp, _vp, _u32]),
+        "clsph_read_order": (_i32, [_vp, _vp, _vp, _u32]),
         "clsph_bounce_capped": (_i32, [_vp, _vp]),
         "clsph_time_steps": (_i32, [_vp, _u32, _vp]),
         "clsph_time_stage": (_i32, [_vp, _i32, _u32, _vp]),
@@ -294,6 +295,14 @@ class State:
         p = np.zeros(self.n, dtype=np.uint32)
         self.ctx.check(lib.clsph_read_sorted(self.h, _ptr(k), _ptr(p), self.n))
         return k, p
+
+    def order(self):
+        """(slot, perm) of the kernels' internal order: internal position k
+        is reference sorted position slot[k] and canonical index perm[k]."""
+        slot = np.zeros(self.n, dtype=np.uint32)
+        perm = np.zeros(self.n, dtype=np.uint32)
+        self.ctx.check(lib.clsph_read_order(self.h, _ptr(slot), _ptr(perm), self.n))
+        return slot, perm
 
     def cell_table(self, count):
         t = np.zeros(count, dtype=np.uint32)

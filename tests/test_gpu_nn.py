@@ -172,7 +172,8 @@ def test_target_and_identities(ctx):
     got = net.target(st.n).astype(np.float64)
     # host restatement: learner's sorted i <- canonical perm[i]; the teacher
     # holds that particle at loc[perm[i]]
-    _, perm = st.sorted()
+    _, perm = st.order()
+    perm = perm.astype(np.int64)
     canon = st.download()
     t_parts = teacher.download()
     loc = np.empty(st.n, np.int64)
