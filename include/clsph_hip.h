@@ -216,6 +216,12 @@ int clsph_nn_read_output(clsph_nn* nn, float* out, uint32_t n);
 int clsph_nn_set_target(clsph_nn* nn, const float* target, uint32_t n);
 int clsph_nn_read_target(clsph_nn* nn, float* target, uint32_t n);
 int clsph_nn_target_from(clsph_nn* nn, clsph_state* learner, clsph_state* teacher, float dt);
+/* Target = the mean over `nsub` plain substeps of each particle's
+ * forces-stage acceleration: `teacher` (a copy of the learner's particles,
+ * identities tracked) is advanced by nsub substeps stage by stage. Collision
+ * impulses, which advection applies, are left out. */
+int clsph_nn_target_mean_acc(clsph_nn* nn, clsph_state* learner, clsph_state* teacher,
+                             uint32_t nsub);
 /* Mean squared error of the last evaluation against the target and its
  * gradient with respect to the parameters (*loss optional), then one Adam
  * step (bias-corrected). */

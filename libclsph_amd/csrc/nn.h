@@ -142,6 +142,13 @@ void cast_params(clsph_nn* m);
 // target[i] = ((vh_T[loc[perm_S[i]]] - vh_S[perm_S[i]]) / dt - g) / A
 void launch_target(clsph_nn* m, clsph_state* learner, clsph_state* teacher, float dt,
                    float gx, float gy, float gz);
+// Advances `teacher` (a copy of the learner's particles with identities
+// reset) by nsub plain substeps stage by stage, summing each particle's
+// forces-stage acceleration by identity; target[i] = (mean - g) / A for the
+// learner's sorted particle i. Collision impulses (applied by advection)
+// are not part of the target.
+void launch_target_mean(clsph_nn* m, clsph_state* learner, clsph_state* teacher, uint32_t nsub,
+                        float gx, float gy, float gz);
 // Per-particle density difference of two states holding the same particles
 // (matched through their identities); res (host, read after a
 // synchronisation) = {max |rho_a - rho_b|, sum |rho_a - rho_b|}.

@@ -1006,6 +1006,35 @@ __global__ __launch_bounds__(256) void k_compare(const uint32_t* __restrict__ pe
   if (threadIdx.x == 0) atomicAdd(res + 1, red[0] + red[1] + red[2] + red[3]);
 }
 
+// Adds the forces stage's acceleration of sorted particle j to its identity's
+// running sum (each identity occurs once per substep).
+__global__ void k_accum_acc(const float4* __restrict__ acc, const uint32_t* __restrict__ perm,
+                            const uint32_t* __restrict__ ids, uint32_t n,
+                            float* __restrict__ sum) {
+  uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  uint32_t c = perm[j];
+  uint32_t id = c < n ? ids[c] : n;
+  if (id >= n) return;
+  float4 a = acc[j];
+  sum[3 * id] += a.x;
+  sum[3 * id + 1] += a.y;
+  sum[3 * id + 2] += a.z;
+}
+
+// target[i] = (sum[perm[i]] / nsub - g) / A for the learner's sorted i.
+__global__ void k_target_mean(const uint32_t* __restrict__ perm, const float* __restrict__ sum,
+                              uint32_t n, float inv_nsub, float gx, float gy, float gz,
+                              float* __restrict__ target) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c = perm[i];
+  if (c >= n) c = 0;
+  target[3 * i] = (sum[3 * c] * inv_nsub - gx) / kAccScale;
+  target[3 * i + 1] = (sum[3 * c + 1] * inv_nsub - gy) / kAccScale;
+  target[3 * i + 2] = (sum[3 * c + 2] * inv_nsub - gz) / kAccScale;
+}
+
 // ---------------------------------------------------------------------------
 inline dim3 grid_for(size_t count, size_t per_block = 256) {
   size_t b = (count + per_block - 1) / per_block;
@@ -1293,6 +1322,29 @@ void adam(clsph_nn* m, float lr, float beta1, float beta2, float eps) {
 void cast_params(clsph_nn* m) {
   hipLaunchKernelGGL(k_cast, grid_for(kParamTotal), dim3(256), 0, m->ctx->stream, m->params,
                      m->wb, (size_t)kParamTotal);
+}
+
+void launch_target_mean(clsph_nn* m, clsph_state* learner, clsph_state* teacher, uint32_t nsub,
+                        float gx, float gy, float gz) {
+  hipStream_t s = m->ctx->stream;
+  uint32_t n = learner->n;
+  float* sum = m->tmp;  // 3 n floats
+  fill(sum, 3 * (size_t)n, 0.f, s);
+  for (uint32_t k = 0; k < nsub; ++k) {
+    launch_grid_params(teacher);
+    launch_hash(teacher);
+    launch_sort(teacher);
+    launch_permute(teacher);
+    launch_cell_table(teacher);
+    launch_density(teacher);
+    launch_forces(teacher);
+    hipLaunchKernelGGL(k_accum_acc, grid_for(n), dim3(256), 0, s, teacher->acc_s, teacher->val_b,
+                       teacher->ids, n, sum);
+    launch_advect(teacher);
+    launch_track_ids(teacher);
+  }
+  hipLaunchKernelGGL(k_target_mean, grid_for(n), dim3(256), 0, s, learner->val_b, sum, n,
+                     1.f / (float)nsub, gx, gy, gz, m->target);
 }
 
 void launch_target(clsph_nn* m, clsph_state* a, clsph_state* t, float dt, float gx, float gy,

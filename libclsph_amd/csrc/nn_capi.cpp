@@ -273,6 +273,26 @@ int clsph_nn_target_from(clsph_nn* m, clsph_state* learner, clsph_state* teacher
   return check_launch(m->ctx);
 }
 
+int clsph_nn_target_mean_acc(clsph_nn* m, clsph_state* learner, clsph_state* teacher,
+                             uint32_t nsub) {
+  int rc = check_state(m, learner);
+  if (rc) return rc;
+  if (!teacher || teacher->n != learner->n || !teacher->ids || teacher->n_scenes != 1)
+    return fail(m->ctx, CLSPH_EINVAL,
+                "the teacher state must hold the learner's particles and track identities");
+  if (learner->stage < kStageSort || nsub == 0)
+    return fail(m->ctx, CLSPH_EINVAL, "the learner must be sorted and nsub positive");
+  float g[3];
+  gravity(learner, g);
+  nn::launch_target_mean(m, learner, teacher, nsub, g[0], g[1], g[2]);
+  rc = check_launch(m->ctx);
+  if (rc) return rc;
+  teacher->acc_zero = true;
+  teacher->fields_canonical = false;
+  teacher->stage = kStageAdvect;
+  return device_errors(teacher);
+}
+
 int clsph_nn_backward(clsph_nn* m, float* loss) {
   if (!m || m->n == 0) return fail(m ? m->ctx : nullptr, CLSPH_EINVAL, "no forward pass to differentiate");
   nn::backward(m);

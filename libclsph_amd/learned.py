@@ -11,11 +11,12 @@ every `gap` substeps the current state is copied into a learner state,
 sorted and given its densities, and the teacher's next 4 plain substeps
 (with particle identities tracked) give each particle's target
 
-    a*_i = (v_half_i(t + 4 dt) - v_half_i(t)) / (4 dt)
+    a*_i = mean over the 4 substeps of the forces stage's a_i
 
-(the acceleration that makes one leapfrog step of 4 dt reproduce the
-plain method's half-step velocity). Each sample is one forward, one
-backward and one Adam step, all on the GPU.
+(gravity plus pressure, viscosity and surface tension along the plain
+trajectory; collision impulses stay with advection, which applies them in
+the learned step too). Each sample is one forward, one backward and one
+Adam step, all on the GPU.
 
 Running (run_learned): macro steps with the network in place of the forces
 stage. Every `judge_every` macro steps the plain method is the judge: the
@@ -131,14 +132,12 @@ def train(net, ctx, trajectories=200, steps=2000, gap=20, minutes=None, lr=1e-3,
         teacher.upload(sc.particles)
         teacher.track_ids(True)
         teacher.step(warm)
-        dt = float(sc.params["time_delta"])
         for smp in range(samples_per_traj):
             learner.copy_from(teacher)
             density_stages(learner)
             teacher.track_ids(True)
-            teacher.step(MACRO)
+            net.target_mean_acc(learner, teacher, MACRO)  # advances the teacher by MACRO
             net.forward(learner)
-            net.target_from(learner, teacher, MACRO * dt)
             frac = min(1.0, k / max(1, total - 1))
             rate = lr_end + 0.5 * (lr - lr_end) * (1.0 + math.cos(math.pi * frac))
             loss = net.backward(read_loss=(smp % 10 == 0))

@@ -99,6 +99,7 @@ def _load():
         "clsph_nn_set_target": (_i32, [_vp, _vp, _u32]),
         "clsph_nn_read_target": (_i32, [_vp, _vp, _u32]),
         "clsph_nn_target_from": (_i32, [_vp, _vp, _vp, ctypes.c_float]),
+        "clsph_nn_target_mean_acc": (_i32, [_vp, _vp, _vp, _u32]),
         "clsph_nn_backward": (_i32, [_vp, _vp]),
         "clsph_nn_read_grad": (_i32, [_vp, _vp, _u32]),
         "clsph_nn_adam": (_i32, [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
@@ -442,6 +443,12 @@ class Network:
 
     def target_from(self, learner, teacher, dt):
         self.ctx.check(lib.clsph_nn_target_from(self.h, learner.h, teacher.h, ctypes.c_float(dt)))
+
+    def target_mean_acc(self, learner, teacher, nsub):
+        """Target: mean forces-stage acceleration over nsub plain substeps of
+        `teacher` (which they advance)."""
+        self.ctx.check(lib.clsph_nn_target_mean_acc(self.h, learner.h, teacher.h, nsub))
+        self.n = learner.n
 
     def backward(self, read_loss=True):
         loss = np.zeros(1, dtype=np.float32)

@@ -266,3 +266,34 @@ def test_fused_inference_matches_training_forward(ctx):
           flush=True)
     assert err.max() <= 1e-2 * scale
     assert rms <= 1e-3
+
+
+def test_mean_acceleration_target(ctx):
+    sc, st = settled_state(ctx, n=2500, steps=25, seed=13)
+    teacher = native.State(ctx, st.n)
+    twin = native.State(ctx, st.n)
+    for s in (teacher, twin):
+        s.set_params(sc.params, sc.terms)
+        s.scene(sc.normals, sc.vertices, sc.indices)
+        s.copy_from(st)
+        s.track_ids(True)
+    net = native.Network(ctx, st.n)
+    net.target_mean_acc(st, teacher, 4)
+    got = net.target(st.n).astype(np.float64)
+    # host restatement: the twin runs the same 4 substeps stage by stage and
+    # its forces-stage accelerations are summed by identity
+    total = np.zeros((st.n, 3))
+    for _ in range(4):
+        for name in ("grid", "hash", "sort", "cell_table", "density", "forces"):
+            twin.stage(name)
+        parts, perm = internal_sorted(twin)
+        ids = twin.ids()
+        total[ids[perm]] += parts["acceleration"][:, :3]
+        twin.stage("advect")
+    _, perm_l = st.order()
+    g = sc.params["constant_acceleration"][:3].astype(np.float64)
+    want = (total[perm_l.astype(np.int64)] / 4.0 - g) / 10.0
+    scale = np.abs(want).max()
+    assert np.abs(got - want).max() <= 1e-5 * scale
+    # the teacher ended where the twin did
+    assert np.array_equal(teacher.download()["position"], twin.download()["position"])
