@@ -222,6 +222,7 @@ def bench_learned(ctx, plain, scene, plain_ms_per_step, macro_steps, judge_every
     at the matched time (max and mean |rho - rho_plain| / rho0), beside the
     literal max |rho / rho0 - 1| of both."""
     from libclsph_amd import learned as lrn
+    from libclsph_amd import native
     if not os.path.exists(lrn.WEIGHTS):
         return {"skipped": f"no trained weights at {lrn.WEIGHTS}"}
     n = plain.n

@@ -4,7 +4,9 @@
 //
 // Network (per particle i, neighbours j; all products bf16 x bf16 -> f32):
 //   nu_i  = [rho_i / rho0 - 1, asinh(p_i / K) / 4]
-//   e_ij  = [(x_i - x_j) / h, (v_i - v_j) / V, q, (1 - q^2)^3, nu_i, nu_j, 0 x 4]
+//   e_ij  = [(x_i - x_j) / h, (v_i - v_j) / V, q, (1 - q^2)^3, nu_i, nu_j,
+//            pressure and viscosity pair accelerations / A, colour-field
+//            gradient term * h and laplacian term * h^2, 0 x 10]
 //   round 1: s1 = silu(e W1 + b1), s2 = silu(s1 W2 + b2),
 //            M1_i = bf16(sum_j s2_ij / C) W3 + (deg_i / C) b3
 //            H_i  = 3-layer SiLU MLP of [M1_i, nu_i, 0 x 14]      (144 -> 128)
@@ -22,7 +24,7 @@ namespace clsph {
 namespace nn {
 
 constexpr int kWidth = 128;
-constexpr int kFeat = 16;
+constexpr int kFeat = 32;
 constexpr int kIn1 = 144;
 constexpr int kIn2 = 256;
 constexpr int kOut = 32;

@@ -255,43 +255,81 @@ __global__ __launch_bounds__(256) void k_graph(const uint4* __restrict__ nbr,
   degc[i] = (float)deg / kAggC;
 }
 
-// The 16 edge inputs of edge (i <- j) in float32 (12 used, 4 zero).
-__device__ inline void edge_features(float4 xi, float4 xj, float4 vi, float4 vj, float2 ni,
-                                     float2 nj, float inv_h, float f[kFeat]) {
+// Constants of the edge inputs (one scene).
+struct EdgeConsts {
+  float h, inv_h, rho0, K, m, mu, c_spiky, c_visc, c_grad, c_lap;
+};
+
+inline EdgeConsts edge_consts(const DevParams& d) {
+  EdgeConsts k;
+  k.h = d.p.h;
+  k.inv_h = 1.f / d.p.h;
+  k.rho0 = d.p.fluid_density;
+  k.K = d.p.K;
+  k.m = d.p.particle_mass;
+  k.mu = d.p.dynamic_viscosity;
+  k.c_spiky = d.t.spiky;
+  k.c_visc = d.t.viscosity;
+  k.c_grad = d.t.poly_6_gradient;
+  k.c_lap = d.t.poly_6_laplacian;
+  return k;
+}
+
+// The 32 edge inputs of edge (i <- j) in float32 (22 used, 10 zero). The
+// pair terms are the ones the forces stage sums (kernels/forces.cl:60-101):
+// pressure and viscosity accelerations in units of A, the colour-field
+// gradient term times h and its laplacian term times h^2.
+__device__ inline void edge_features(float4 xi, float4 xj, float4 vi, float4 vj, float rhoi,
+                                     float rhoj, float pi, float pj, const EdgeConsts& k,
+                                     float f[kFeat]) {
   float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-  float q = sqrtf(dx * dx + dy * dy + dz * dz) * inv_h;
+  float d2 = dx * dx + dy * dy + dz * dz;
+  float r = sqrtf(d2);
+  float q = r * k.inv_h;
   float w = 1.f - q * q;
-  f[0] = dx * inv_h, f[1] = dy * inv_h, f[2] = dz * inv_h;
+  float2 ni = node_in(rhoi, pi, k.rho0, k.K), nj = node_in(rhoj, pj, k.rho0, k.K);
+  f[0] = dx * k.inv_h, f[1] = dy * k.inv_h, f[2] = dz * k.inv_h;
   f[3] = (vi.x - vj.x) / kVelScale, f[4] = (vi.y - vj.y) / kVelScale;
   f[5] = (vi.z - vj.z) / kVelScale;
   f[6] = q, f[7] = w * w * w;
   f[8] = ni.x, f[9] = ni.y, f[10] = nj.x, f[11] = nj.y;
-  f[12] = f[13] = f[14] = f[15] = 0.f;
+  const float hr = k.h - r, t = k.h * k.h - d2;
+  const float ps = (pi / (rhoi * rhoi) + pj / (rhoj * rhoj)) * k.m;
+  const float g = r >= 1e-7f ? ps * k.c_spiky * hr * hr / r : 0.f;
+  f[12] = -g * dx / kAccScale, f[13] = -g * dy / kAccScale, f[14] = -g * dz / kAccScale;
+  const float wj = k.m / rhoj;
+  const float vw = k.mu / rhoi * wj * k.c_visc * hr / kAccScale;
+  f[15] = (vj.x - vi.x) * vw, f[16] = (vj.y - vi.y) * vw, f[17] = (vj.z - vi.z) * vw;
+  const float gw = wj * k.c_grad * t * t * k.h;
+  f[18] = gw * dx, f[19] = gw * dy, f[20] = gw * dz;
+  f[21] = wj * k.c_lap * t * (3.f * k.h * k.h - 7.f * d2) * k.h * k.h;
+  for (int c = 22; c < kFeat; ++c) f[c] = 0.f;
 }
 
-// Edge inputs e (16 bf16 per edge) and node inputs nu.
+// Edge inputs e (32 bf16 per edge) and node inputs nu.
 __global__ __launch_bounds__(256) void k_edge_inputs(
     const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float* __restrict__ rho, const float* __restrict__ prs,
     const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint32_t e, uint32_t n,
-    float h, float rho0, float K, uint16_t* __restrict__ E1, float* __restrict__ nu) {
+    EdgeConsts kc, uint16_t* __restrict__ E1, float* __restrict__ nu) {
   uint32_t k = blockIdx.x * 256 + threadIdx.x;
   if (k < n) {
-    float2 a = node_in(rho[k], prs[k], rho0, K);
+    float2 a = node_in(rho[k], prs[k], kc.rho0, kc.K);
     nu[2 * k] = a.x;
     nu[2 * k + 1] = a.y;
   }
   if (k >= e) return;
   uint32_t i = dst[k], j = src[k];
   float f[kFeat];
-  edge_features(pos[i], pos[j], vel[i], vel[j], node_in(rho[i], prs[i], rho0, K),
-                node_in(rho[j], prs[j], rho0, K), 1.f / h, f);
-  uint32_t packed[8];
-  for (int t = 0; t < 8; ++t)
-    packed[t] = (uint32_t)to_bf16(f[2 * t]) | ((uint32_t)to_bf16(f[2 * t + 1]) << 16);
+  edge_features(pos[i], pos[j], vel[i], vel[j], rho[i], rho[j], prs[i], prs[j], kc, f);
   u32x4* o = (u32x4*)(E1 + (size_t)k * kFeat);
-  o[0] = u32x4{packed[0], packed[1], packed[2], packed[3]};
-  o[1] = u32x4{packed[4], packed[5], packed[6], packed[7]};
+  for (int q = 0; q < kFeat / 8; ++q) {
+    uint32_t packed[4];
+    for (int t = 0; t < 4; ++t)
+      packed[t] = (uint32_t)to_bf16(f[8 * q + 2 * t]) |
+                  ((uint32_t)to_bf16(f[8 * q + 2 * t + 1]) << 16);
+    o[q] = u32x4{packed[0], packed[1], packed[2], packed[3]};
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -321,8 +359,8 @@ template <bool ROUND2>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_edge_fused(
     const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float* __restrict__ rho, const float* __restrict__ prs,
-    const uint4* __restrict__ nbr, const uint32_t* __restrict__ count, uint32_t n, float h,
-    float rho0, float K, const uint16_t* __restrict__ W1, const float* __restrict__ b1,
+    const uint4* __restrict__ nbr, const uint32_t* __restrict__ count, uint32_t n,
+    EdgeConsts kc, const uint16_t* __restrict__ W1, const float* __restrict__ b1,
     const uint16_t* __restrict__ W2, const float* __restrict__ b2,
     const uint16_t* __restrict__ PA, const uint16_t* __restrict__ PB,
     uint16_t* __restrict__ S, float* __restrict__ degc, float* __restrict__ nu) {
@@ -359,11 +397,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
   const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
   const uint32_t waves = gridDim.x * 4;
-  const float inv_h = 1.f / h;
   for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += waves) {
     const uint32_t deg = min(count[i], kMaxEdges);
     const float4 xi = pos[i], vi = vel[i];
-    const float2 ni = node_in(rho[i], prs[i], rho0, K);
+    const float rhoi = rho[i], pi = prs[i];
+    const float2 ni = node_in(rhoi, pi, kc.rho0, kc.K);
     float pa[8][4];
     for (int ob = 0; ob < 8; ++ob)
       for (int r = 0; r < 4; ++r) pa[ob][r] = bs[16 * ob + 4 * g + r];
@@ -389,9 +427,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       }
       // B operand of layer 1: features 8 g .. 8 g + 7 of this lane's edge
       float f[kFeat];
-      edge_features(xi, pos[j], vi, vel[j], ni, node_in(rho[j], prs[j], rho0, K), inv_h, f);
+      edge_features(xi, pos[j], vi, vel[j], rhoi, rho[j], pi, prs[j], kc, f);
       float sel8[8];
-      for (int t = 0; t < 8; ++t) sel8[t] = g == 0 ? f[t] : (g == 1 ? f[8 + t] : 0.f);
+      for (int t = 0; t < 8; ++t)
+        sel8[t] = g == 0 ? f[t] : (g == 1 ? f[8 + t] : (g == 2 ? f[16 + t] : f[24 + t]));
       const bf16x8 eb = __builtin_bit_cast(bf16x8, pack8(sel8));
       // layer 1 (+ bias, node terms), SiLU, packed into layer 2's B operands
       float pb[8][4];
@@ -1068,8 +1107,7 @@ void launch_edge_inputs(clsph_nn* m, clsph_state* st) {
                      m->src, m->dst, m->degc);
   uint32_t cnt = e > n ? e : n;
   hipLaunchKernelGGL(k_edge_inputs, grid_for(cnt), dim3(256), 0, s, st->pos_s, st->vel_s,
-                     st->rho_s, st->prs_s, m->src, m->dst, e, n, d.p.h, d.p.fluid_density,
-                     d.p.K, m->E1, m->nu);
+                     st->rho_s, st->prs_s, m->src, m->dst, e, n, edge_consts(d), m->E1, m->nu);
 }
 
 static inline const float* P(const clsph_nn* m, int id) { return m->params + param_offset(id); }
@@ -1166,8 +1204,8 @@ void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float gx, float gy
   if (blocks == 0) blocks = 1;
   if (ev) (void)hipEventRecord(ev[0], s);
   hipLaunchKernelGGL((k_edge_fused<false>), dim3(blocks), dim3(256), 0, s, st->pos_s, st->vel_s,
-                     st->rho_s, st->prs_s, st->nbr, st->nbr_count, n, d.p.h, d.p.fluid_density,
-                     d.p.K, W(m, E1_W1), P(m, E1_B1), W(m, E1_W2), P(m, E1_B2),
+                     st->rho_s, st->prs_s, st->nbr, st->nbr_count, n, edge_consts(d),
+                     W(m, E1_W1), P(m, E1_B1), W(m, E1_W2), P(m, E1_B2),
                      (const uint16_t*)nullptr, (const uint16_t*)nullptr, m->S[0], m->degc, m->nu);
   if (ev) (void)hipEventRecord(ev[1], s);
   // node part of round 1 (fused)
@@ -1195,8 +1233,8 @@ void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float gx, float gy
                      m->PB);
   if (ev) (void)hipEventRecord(ev[2], s);
   hipLaunchKernelGGL((k_edge_fused<true>), dim3(blocks), dim3(256), 0, s, st->pos_s, st->vel_s,
-                     st->rho_s, st->prs_s, st->nbr, st->nbr_count, n, d.p.h, d.p.fluid_density,
-                     d.p.K, W(m, E2_W1), P(m, E2_B1), W(m, E2_W2), P(m, E2_B2), m->PA, m->PB,
+                     st->rho_s, st->prs_s, st->nbr, st->nbr_count, n, edge_consts(d),
+                     W(m, E2_W1), P(m, E2_B1), W(m, E2_W2), P(m, E2_B2), m->PA, m->PB,
                      m->S[1], m->degc, m->nu);
   if (ev) (void)hipEventRecord(ev[3], s);
   hipLaunchKernelGGL(k_node2, dim3(nblocks), dim3(512), kLdsNode2, s, m->S[1], m->Hb, m->degc, n,

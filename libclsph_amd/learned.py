@@ -44,13 +44,13 @@ DENSITY_STAGES = ("grid", "hash", "sort", "cell_table", "density")
 
 # Parameter vector layout of the network (libclsph_amd/csrc/nn.h, enum Param):
 # (name, shape, used input rows) in storage order, row-major inputs x outputs.
-W, FEAT, IN1, IN2, OUT = 128, 16, 144, 256, 32
+W, FEAT, IN1, IN2, OUT = 128, 32, 144, 256, 32
 LAYOUT = [
-    ("e1.w1", (FEAT, W), 12), ("e1.b1", (W,), 0), ("e1.w2", (W, W), W), ("e1.b2", (W,), 0),
+    ("e1.w1", (FEAT, W), 22), ("e1.b1", (W,), 0), ("e1.w2", (W, W), W), ("e1.b2", (W,), 0),
     ("e1.w3", (W, W), W), ("e1.b3", (W,), 0),
     ("n1.w1", (IN1, W), W + 2), ("n1.b1", (W,), 0), ("n1.w2", (W, W), W), ("n1.b2", (W,), 0),
     ("n1.w3", (W, W), W), ("n1.b3", (W,), 0),
-    ("e2.w1", (FEAT, W), 12), ("e2.b1", (W,), 0), ("e2.wa", (W, W), W), ("e2.wb", (W, W), W),
+    ("e2.w1", (FEAT, W), 22), ("e2.b1", (W,), 0), ("e2.wa", (W, W), W), ("e2.wb", (W, W), W),
     ("e2.w2", (W, W), W), ("e2.b2", (W,), 0), ("e2.w3", (W, W), W), ("e2.b3", (W,), 0),
     ("n2.w1", (IN2, W), IN2), ("n2.b1", (W,), 0), ("n2.w2", (W, W), W), ("n2.b2", (W,), 0),
     ("n2.w3", (W, OUT), W), ("n2.b3", (OUT,), 0),

@@ -11,7 +11,13 @@ excluded, at most 128 neighbours per particle) with two rounds:
 
   node inputs   nu_i = [rho_i / rho0 - 1, asinh(p_i / K) / 4]
   edge inputs   e_ij = [(x_i - x_j) / h, (v_i - v_j) / V, q, (1 - q^2)^3,
-                        nu_i, nu_j, 0 0 0 0]            q = |x_i - x_j| / h
+                        nu_i, nu_j,                     q = |x_i - x_j| / h
+                        pressure and viscosity pair accelerations / A,
+                        colour-gradient pair term * h, colour-laplacian term * h^2,
+                        0 x 10]
+                the pair terms are those the forces kernel sums
+                (kernels/forces.cl:60-101), from positions, velocities,
+                densities and pressures only
   round 1       s1 = silu(e W1 + b1); s2 = silu(s1 W2 + b2)
                 M1_i = (sum_j s2_ij / C) W3 + (deg_i / C) b3
                 H_i  = MLP_n1([M1_i, nu_i, 0...])        (3 layers, 144 -> 128)
@@ -35,7 +41,8 @@ central finite differences of `forward`.
 import numpy as np
 
 WIDTH = 128
-FEAT = 16          # edge input features (12 used, 4 zero)
+FEAT = 32          # edge input features (22 used, 10 zero)
+FEAT_USED = 22
 NODE_IN1 = 144     # node MLP 1 input: M1 (128) + nu (2) + zeros
 NODE_IN2 = 256     # node MLP 2 input: H (128) + M2 (128)
 OUT = 32           # node MLP 2 output (3 used)
@@ -114,11 +121,15 @@ def node_inputs(rho, prs, rho0, K):
     return nu
 
 
-def edge_inputs(x, v, nu, src, dst, h):
+def edge_inputs(x, v, nu, src, dst, h, rho=None, prs=None, consts=None):
+    """Edge inputs; the pair terms (columns 12..21) need rho, prs and
+    consts = (particle_mass, dynamic_viscosity, spiky, viscosity,
+    poly_6_gradient, poly_6_laplacian) and are zero without them."""
     x = np.asarray(x, np.float64)
     v = np.asarray(v, np.float64)
     d = x[dst] - x[src]
-    q = np.sqrt((d * d).sum(1)) / h
+    r = np.sqrt((d * d).sum(1))
+    q = r / h
     e = np.zeros((len(src), FEAT))
     e[:, 0:3] = d / h
     e[:, 3:6] = (v[dst] - v[src]) / VEL_SCALE
@@ -126,6 +137,24 @@ def edge_inputs(x, v, nu, src, dst, h):
     e[:, 7] = (1.0 - q * q) ** 3
     e[:, 8:10] = nu[dst]
     e[:, 10:12] = nu[src]
+    if consts is not None:
+        m, mu, c_spiky, c_visc, c_grad, c_lap = (float(c) for c in consts)
+        rho = np.asarray(rho, np.float64)
+        prs = np.asarray(prs, np.float64)
+        hr = h - r
+        t = h * h - r * r
+        # pressure: -(p_i/rho_i^2 + p_j/rho_j^2) m c_spiky (h - r)^2 (x_i - x_j) / r
+        ps = (prs[dst] / rho[dst] ** 2 + prs[src] / rho[src] ** 2) * m
+        g = np.where(r >= 1e-7, ps * c_spiky * hr * hr / np.maximum(r, 1e-30), 0.0)
+        e[:, 12:15] = -g[:, None] * d / ACC_SCALE
+        # viscosity: mu / rho_i (v_j - v_i) (m / rho_j) c_visc (h - r)
+        vw = (m / rho[src]) * c_visc * hr
+        e[:, 15:18] = (mu / rho[dst] * vw)[:, None] * (v[src] - v[dst]) / ACC_SCALE
+        # colour field: gradient (m / rho_j) c_grad t^2 (x_i - x_j) and laplacian
+        # (m / rho_j) c_lap t (3 h^2 - 7 r^2), scaled by h and h^2
+        w = m / rho[src]
+        e[:, 18:21] = (w * c_grad * t * t * h)[:, None] * d
+        e[:, 21] = w * c_lap * t * (3 * h * h - 7 * r * r) * h * h
     return e
 
 
@@ -135,7 +164,8 @@ def segsum(vals, idx, n):
     return out
 
 
-def forward(flat, x, v, rho, prs, src, dst, deg, h, rho0, K, bf16_round=False, keep=False):
+def forward(flat, x, v, rho, prs, src, dst, deg, h, rho0, K, consts=None, bf16_round=False,
+            keep=False):
     """Network output `out` (N x 32; columns 0..2 are the acceleration
     residual in units of A). With keep=True also returns the intermediates
     the backward pass uses."""
@@ -144,7 +174,7 @@ def forward(flat, x, v, rho, prs, src, dst, deg, h, rho0, K, bf16_round=False, k
     n = len(rho)
     degc = np.asarray(deg, np.float64)[:, None] / AGG_C
     nu = node_inputs(rho, prs, rho0, K)
-    e = R(edge_inputs(x, v, nu, src, dst, h))
+    e = R(edge_inputs(x, v, nu, src, dst, h, rho, prs, consts))
     c = {"e": e, "nu": nu}
     # round 1 edges
     z1 = e @ R(P["e1.w1"]) + P["e1.b1"]
@@ -242,7 +272,7 @@ def init_params(seed=0):
     are used, zero biases, zero rows for the padded inputs and a last layer
     scaled by 0.1."""
     rng = np.random.default_rng(seed)
-    used_in = {"e1.w1": 12, "e2.w1": 12, "n1.w1": WIDTH + 2}
+    used_in = {"e1.w1": FEAT_USED, "e2.w1": FEAT_USED, "n1.w1": WIDTH + 2}
     P = {}
     for name, shape in LAYOUT:
         if len(shape) == 1:

@@ -122,37 +122,38 @@ def test_judged_run_counts_fallbacks(ctx):
     assert mx == 0.0
 
 
-def fp64_trajectory(sc, parts, steps):
-    """FP64 oracle trajectory: grid and sort in FP32 (the reference's cell
-    ids) on positions rounded from FP64, density, forces and advection in
-    FP64 (oracle *_f64). Particle identities ride in position[3]. Returns
-    the densities at the end, indexed by particle identity."""
+def oracle_trajectory(sc, parts, steps, f64=True):
+    """Oracle trajectory driven stage by stage: grid and cell ids in FP32 (the
+    reference's) on positions rounded from the state, the stable sort as a
+    stable argsort by cell id (the oracle's sort is pinned to it), then
+    density, forces and advection in FP64 (oracle *_f64) or FP32. Returns
+    the densities at the end, indexed by the particles' initial positions."""
     p = sc.params.copy()
-    p32 = parts.copy()
-    p32["position"][:, 3] = np.arange(len(parts), dtype=np.float32)
-    p64 = oracle.to_f64(p32)
-    by_id = p64.copy()
+    by_id = oracle.to_f64(parts) if f64 else parts.copy()
 
     def sorted_state():
-        cur = np.zeros_like(p32)
+        cur = np.zeros_like(parts)
         for f in ("position", "velocity", "intermediate_velocity", "acceleration"):
             cur[f] = by_id[f].astype(np.float32)
-        cur["position"][:, 3] = np.arange(len(parts), dtype=np.float32)
         oracle.grid_params(cur, p)
-        s32 = oracle.sort_particles(oracle.locate_in_grid(cur, p))
-        ids = s32["position"][:, 3].astype(np.int64)
-        table = oracle.cell_table(s32, int(p["grid_cell_count"]))
-        s64 = by_id[ids].copy()
-        return ids, table, s64
+        hashed = oracle.locate_in_grid(cur, p)
+        ids = np.argsort(hashed["grid_index"], kind="stable")
+        table = oracle.cell_table(hashed[ids], int(p["grid_cell_count"]))
+        cur = by_id[ids].copy()
+        cur["grid_index"] = hashed["grid_index"][ids]  # the kernels look neighbours up by it
+        return ids, table, cur
 
+    dens = oracle.density_pressure_f64 if f64 else oracle.density_pressure
+    forc = oracle.forces_f64 if f64 else oracle.forces
+    adv = oracle.advection_collision_f64 if f64 else oracle.advection_collision
     for _ in range(steps):
-        ids, table, s64 = sorted_state()
-        d = oracle.density_pressure_f64(s64, p, sc.terms, table)
-        f = oracle.forces_f64(d, p, sc.terms, table)
-        a, _ = oracle.advection_collision_f64(f, p, sc.normals, sc.vertices, sc.indices)
+        ids, table, cur = sorted_state()
+        d = dens(cur, p, sc.terms, table)
+        f = forc(d, p, sc.terms, table)
+        a, _ = adv(f, p, sc.normals, sc.vertices, sc.indices)
         by_id[ids] = a
-    ids, table, s64 = sorted_state()
-    d = oracle.density_pressure_f64(s64, p, sc.terms, table)
+    ids, table, cur = sorted_state()
+    d = dens(cur, p, sc.terms, table)
     rho = np.empty(len(parts))
     rho[ids] = d["density"]
     return rho
@@ -173,7 +174,7 @@ def test_error_against_fp64_oracle(ctx):
     net.set_params(np.load(learned.WEIGHTS))
     plain.step(macro * learned.MACRO)
     learner.learned_step(net, macro)
-    ref = fp64_trajectory(sc, sc.particles, macro * learned.MACRO)
+    ref = oracle_trajectory(sc, sc.particles, macro * learned.MACRO, f64=True)
     out = {}
     for name, st in (("plain_f32", plain), ("learned", learner)):
         learned.density_stages(st)

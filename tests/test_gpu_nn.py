@@ -73,10 +73,13 @@ def oracle_inputs(sc, st, net):
     off, src = net.graph()
     deg = np.diff(off.astype(np.int64))
     dst = np.repeat(np.arange(st.n), deg)
-    p = sc.params
+    p, t = sc.params, sc.terms
+    consts = (p["particle_mass"], p["dynamic_viscosity"], t["spiky"], t["viscosity"],
+              t["poly_6_gradient"], t["poly_6_laplacian"])
     return dict(x=s["position"][:, :3], v=s["velocity"][:, :3], rho=s["density"],
                 prs=s["pressure"], src=src.astype(np.int64), dst=dst, deg=deg,
-                h=float(p["h"]), rho0=float(p["fluid_density"]), K=float(p["K"]))
+                h=float(p["h"]), rho0=float(p["fluid_density"]), K=float(p["K"]),
+                consts=tuple(float(np.float32(c)) for c in consts))
 
 
 def test_graph_is_the_density_neighbourhood(ctx):

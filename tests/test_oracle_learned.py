@@ -27,14 +27,20 @@ def tiny_graph(n=14, seed=0, h=0.01):
         d = np.linalg.norm(x - x[i], axis=1)
         lists.append([j for j in range(n) if j != i and d[j] < h])
     off, src, dst, deg = L.graph_from_lists(lists)
-    return dict(x=x, v=v, rho=rho, prs=prs, src=src, dst=dst, deg=deg, h=h, rho0=1000.0, K=3.0)
+    # (particle mass, viscosity, spiky, viscosity kernel, poly6 gradient, poly6 laplacian)
+    # of spacing h/2, as the dam breaks use them
+    m = 1000.0 * (h / 2) ** 3
+    consts = (m, 3.5e-3, -45 / (np.pi * h ** 6), 45 / (np.pi * h ** 6),
+              -945 / (32 * np.pi * h ** 9), -945 / (32 * np.pi * h ** 9))
+    return dict(x=x, v=v, rho=rho, prs=prs, src=src, dst=dst, deg=deg, h=h, rho0=1000.0, K=3.0,
+                consts=consts)
 
 
 def test_param_layout():
-    assert L.param_count() == 208288
+    assert L.param_count() == 212384
     flat = L.init_params(0)
     P = L.unpack(flat)
-    assert np.all(P["e1.w1"][12:] == 0) and np.all(P["e2.w1"][12:] == 0)
+    assert np.all(P["e1.w1"][22:] == 0) and np.all(P["e2.w1"][22:] == 0)
     assert np.all(P["n1.w1"][130:] == 0) and np.all(P["n2.w3"][:, 3:] == 0)
     assert np.array_equal(L.pack(P), flat)
 
@@ -69,7 +75,7 @@ def test_padded_parameters_get_zero_gradient():
     target = np.ones((len(g["rho"]), 3))
     out, c = L.forward(flat, keep=True, **g)
     G = L.unpack(L.backward(flat, c, target, g["src"], g["dst"], g["deg"]))
-    assert np.all(G["e1.w1"][12:] == 0) and np.all(G["e2.w1"][12:] == 0)
+    assert np.all(G["e1.w1"][22:] == 0) and np.all(G["e2.w1"][22:] == 0)
     assert np.all(G["n1.w1"][130:] == 0)
     assert np.all(G["n2.w3"][:, 3:] == 0) and np.all(G["n2.b3"][3:] == 0)
 
@@ -87,9 +93,19 @@ def test_sum_then_project_equals_per_edge_layer():
             if j == i or np.linalg.norm(g["x"][i] - g["x"][j]) >= g["h"]:
                 continue
             d = g["x"][i] - g["x"][j]
-            q = np.sqrt(d @ d) / g["h"]
-            e = np.concatenate([d / g["h"], g["v"][i] - g["v"][j], [q, (1 - q * q) ** 3],
-                                nu[i], nu[j], np.zeros(4)])
+            r = np.sqrt(d @ d)
+            q = r / g["h"]
+            h = g["h"]
+            m, mu, c_spiky, c_visc, c_grad, c_lap = g["consts"]
+            ps = (g["prs"][i] / g["rho"][i] ** 2 + g["prs"][j] / g["rho"][j] ** 2) * m
+            pres = -ps * c_spiky * (h - r) ** 2 / r * d / L.ACC_SCALE
+            visc = mu / g["rho"][i] * (m / g["rho"][j]) * c_visc * (h - r) * \
+                (g["v"][j] - g["v"][i]) / L.ACC_SCALE
+            t = h * h - r * r
+            grad = (m / g["rho"][j]) * c_grad * t * t * h * d
+            lap = (m / g["rho"][j]) * c_lap * t * (3 * h * h - 7 * r * r) * h * h
+            e = np.concatenate([d / h, g["v"][i] - g["v"][j], [q, (1 - q * q) ** 3],
+                                nu[i], nu[j], pres, visc, grad, [lap], np.zeros(10)])
             s1 = L.silu(e @ P["e1.w1"] + P["e1.b1"])
             s2 = L.silu(s1 @ P["e1.w2"] + P["e1.b2"])
             M1[i] += (s2 @ P["e1.w3"] + P["e1.b3"]) / L.AGG_C
@@ -134,3 +150,48 @@ def test_product_layout_and_init_match_the_oracle():
     from libclsph_amd import learned as product
     assert [(n, s) for n, s, _ in product.LAYOUT] == L.LAYOUT
     assert np.array_equal(product.init_params(3), L.init_params(3).astype(np.float32))
+
+
+def test_pair_features_sum_to_the_forces_stage():
+    """Summed over each particle's neighbours, the pair-term inputs rebuild
+    the oracle's FP64 forces-stage acceleration (pressure and viscosity
+    directly; surface tension from the colour-field gradient and laplacian
+    with the self term and the threshold)."""
+    import oracle
+    from libclsph_amd import scenes
+    sc = scenes.dam_break(512, seed=3)
+    st32, _, _, p = oracle.run(sc.particles, sc.params, sc.terms, sc.normals, sc.vertices,
+                               sc.indices, 30)
+    r = oracle.stages(st32, p, sc.terms, sc.normals, sc.vertices, sc.indices)
+    s64 = oracle.to_f64(r.sorted)
+    d64 = oracle.density_pressure_f64(s64, r.params, sc.terms, r.table)
+    f64 = oracle.forces_f64(d64, r.params, sc.terms, r.table)
+    x = d64["position"][:, :3]
+    v = d64["velocity"][:, :3]
+    rho, prs = d64["density"], d64["pressure"]
+    h = float(p["h"])
+    t = sc.terms
+    consts = (float(p["particle_mass"]), float(p["dynamic_viscosity"]), float(t["spiky"]),
+              float(t["viscosity"]), float(t["poly_6_gradient"]), float(t["poly_6_laplacian"]))
+    n = len(rho)
+    lists = []
+    for i in range(n):
+        d = np.linalg.norm(x - x[i], axis=1)
+        lists.append([j for j in np.nonzero(d < h)[0] if j != i])
+    off, src, dst, deg = L.graph_from_lists(lists)
+    assert deg.max() < L.MAX_EDGES
+    nu = L.node_inputs(rho, prs, float(p["fluid_density"]), float(p["K"]))
+    e = L.edge_inputs(x, v, nu, src, dst, h, rho, prs, consts)
+    m, c_lap = consts[0], consts[5]
+    a = L.ACC_SCALE * L.segsum(e[:, 12:18], dst, n)
+    a = a[:, 0:3] + a[:, 3:6]
+    normal = L.segsum(e[:, 18:21], dst, n) / h
+    lap = m / rho * c_lap * h * h * 3 * h * h + L.segsum(e[:, 21:22], dst, n)[:, 0] / (h * h)
+    nl = np.linalg.norm(normal, axis=1)
+    on = nl > float(p["surface_tension_threshold"])
+    st = np.zeros((n, 3))
+    st[on] = (-float(p["surface_tension"]) * lap[on] / nl[on])[:, None] * normal[on]
+    a += st / rho[:, None] + p["constant_acceleration"][:3].astype(np.float64)
+    ref = f64["acceleration"][:, :3]
+    assert on.any() and (~on).any()
+    assert np.abs(a - ref).max() <= 1e-9 * np.abs(ref).max()
