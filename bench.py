@@ -226,11 +226,15 @@ def bench_learned(ctx, plain, scene, plain_ms_per_step, macro_steps, judge_every
     if not os.path.exists(lrn.WEIGHTS):
         return {"skipped": f"no trained weights at {lrn.WEIGHTS}"}
     n = plain.n
+    weights = np.load(lrn.WEIGHTS)
+    if weights.shape != (native.lib.clsph_nn_param_count(),):
+        return {"skipped": f"{lrn.WEIGHTS} holds {weights.size} parameters, the network "
+                           f"{native.lib.clsph_nn_param_count()}"}
     dt = float(scene.params["time_delta"])
     learner = lrn.make_state(ctx, scene, dt=lrn.MACRO * dt)
     teacher = lrn.make_state(ctx, scene)
     net = native.Network(ctx, n, e_cap=1024)
-    net.set_params(np.load(lrn.WEIGHTS))
+    net.set_params(weights)
     plain.track_ids(True)
     learner.track_ids(True)
     teacher.track_ids(True)
