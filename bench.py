@@ -54,6 +54,23 @@ def stage_bytes(n, cells):
     }
 
 
+STAGE_KERNEL = {"hash": "clsph::k_hash", "density": "clsph::k_density",
+                "forces": "clsph::k_forces", "advect": "clsph::k_advect"}
+
+
+def measured_traffic(stage, workload):
+    """HBM bytes per launch of the stage's kernel from the committed rocprof
+    counter summary (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE runs,
+    scaled by a copy kernel of known size), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    if workload != "dam_break" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    v = t["kernels"].get(STAGE_KERNEL.get(stage, ""))
+    return float(v) if v is not None else None
+
+
 def cpu_baseline(scene, label, budget_s=20.0, max_steps=3):
     """The oracle (plain C port of the reference substep, OpenMP over the
     host cores, collisions tested against every triangle as the reference
@@ -192,7 +209,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom, args.workload) if n == 1048576 else None,
         },
     }
     if learned is not None:

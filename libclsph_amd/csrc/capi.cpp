@@ -318,10 +318,15 @@ int alloc_state(clsph_ctx* ctx, uint32_t n_scenes, const uint32_t* caps, bool fi
   uint32_t n_cap = st->n_cap;
   st->n_tiles_cap = sort_tiles_for(n_cap ? n_cap : 1);
   int rc = CLSPH_OK;
-  float4** f4[] = {&st->pos, &st->vel, &st->vh, &st->pos_s, &st->vel_s, &st->vh_s,
+  float4** f4[] = {&st->pos, &st->vel, &st->vh, &st->vel_s, &st->vh_s,
                    &st->acc_s, &st->fa, &st->fb};
   for (auto p : f4)
     if (rc == CLSPH_OK) rc = dalloc(ctx, p, n_cap);
+  // sorted positions carry 4 padding entries: density loads candidates in
+  // groups of 4 without clamping the last group to the run's end
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->pos_s, (size_t)n_cap + 4);
+  if (rc == CLSPH_OK)
+    HIPCK(ctx, hipMemset(st->pos_s + n_cap, 0, 4 * sizeof(float4)));
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->rho_s, n_cap);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->prs_s, n_cap);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->nbr, (size_t)(kMaxNeighbours / 4) * n_cap);

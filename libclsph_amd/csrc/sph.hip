@@ -157,12 +157,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_density(const float4* __restrict_
   half_cell_frac(xi, dp, hc, fr);
   float acc = 0.f;
   uint32_t cnt = 0;
-  uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;  // pending list entries
+  // pending list entries, a shift register: the newest is p3
+  uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
   for_each_run(hc, fr, dp->key_base, table, F, n, [&](uint32_t s, uint32_t e) {
     for (uint32_t j0 = s; j0 < e; j0 += 4) {
+      // pos is padded by 4 entries past the particles, so the group's
+      // loads need no clamp; entries at or past e are masked out below
+      const float4* pj = pos + j0;
       float4 c[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) c[u] = pos[j0 + u < e ? j0 + u : e - 1];
+      for (int u = 0; u < 4; ++u) c[u] = pj[u];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint32_t j = j0 + u;
@@ -173,19 +177,23 @@ __global__ __launch_bounds__(kMaxBlock) void k_density(const float4* __restrict_
           acc += t * t * t;
           const bool app = j != i;
           const bool keep = app && cnt < kMaxNeighbours;
-          const uint32_t q = cnt & 3u;
-          p0 = keep && q == 0 ? j : p0;
-          p1 = keep && q == 1 ? j : p1;
-          p2 = keep && q == 2 ? j : p2;
-          p3 = keep && q == 3 ? j : p3;
-          if (keep && q == 3) nbr[(size_t)(cnt >> 2) * n + i] = make_uint4(p0, p1, p2, p3);
+          p0 = keep ? p1 : p0;
+          p1 = keep ? p2 : p1;
+          p2 = keep ? p3 : p2;
+          p3 = keep ? j : p3;
+          if (keep && (cnt & 3u) == 3u) nbr[(size_t)(cnt >> 2) * n + i] = make_uint4(p0, p1, p2, p3);
           cnt += app ? 1u : 0u;
         }
       }
     }
   });
-  if (cnt < kMaxNeighbours && (cnt & 3u))
-    nbr[(size_t)(cnt >> 2) * n + i] = make_uint4(p0, p1, p2, p3);
+  const uint32_t r = cnt & 3u;
+  if (cnt < kMaxNeighbours && r) {
+    // the r pending entries are p(4-r) .. p3, oldest first
+    uint4 last = r == 1 ? make_uint4(p3, 0u, 0u, 0u)
+                        : (r == 2 ? make_uint4(p2, p3, 0u, 0u) : make_uint4(p1, p2, p3, 0u));
+    nbr[(size_t)(cnt >> 2) * n + i] = last;
+  }
   nbr_count[i] = cnt;
   // density = sum m * poly6 * (h^2 - r^2)^3 ; Tait pressure (sph.cl:36-39)
   float m = dp->p.particle_mass;
