@@ -97,22 +97,11 @@ __host__ __device__ inline uint32_t bitlen(uint32_t v) {
   return b;
 }
 
-// Work-groups of a launch are dispatched to the 8 XCDs round-robin (work-group
-// b runs on XCD b % 8), each XCD with its own L2 cache. The logical work-group
-// of b gives every XCD one contiguous range of logical work-groups, so the
-// particles an XCD handles are neighbours in sorted order, and in space, and
-// their gathers share its L2. A bijection of [0, nb).
-constexpr uint32_t kXcds = 8;
-__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
-  const uint32_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds, k = b / kXcds;
-  return x * q + (x < r ? x : r) + k;
-}
-
 // The particle a thread of a per-particle kernel handles, or UINT32_MAX when
 // the thread is past the end of its scene.
 __device__ inline uint32_t particle_of(const BlockDesc* __restrict__ blocks,
                                        const DevParams* __restrict__ dps, uint32_t& scene) {
-  BlockDesc b = blocks[xcd_block(blockIdx.x, gridDim.x)];
+  BlockDesc b = blocks[blockIdx.x];
   scene = b.scene;
   uint32_t i = b.first + threadIdx.x;
   const DevParams& d = dps[b.scene];

@@ -257,7 +257,7 @@ __global__ __launch_bounds__(256) void k_graph(const uint4* __restrict__ nbr,
 
 // Constants of the edge inputs (one scene).
 struct EdgeConsts {
-  float h, inv_h, rho0, K, m, mu, c_spiky, c_visc, c_grad, c_lap;
+  float h, inv_h, rho0, K, m, mu_over_m, c_spiky, c_visc, c_grad, c_lap;
 };
 
 inline EdgeConsts edge_consts(const DevParams& d) {
@@ -267,7 +267,7 @@ inline EdgeConsts edge_consts(const DevParams& d) {
   k.rho0 = d.p.fluid_density;
   k.K = d.p.K;
   k.m = d.p.particle_mass;
-  k.mu = d.p.dynamic_viscosity;
+  k.mu_over_m = d.p.dynamic_viscosity / d.p.particle_mass;
   k.c_spiky = d.t.spiky;
   k.c_visc = d.t.viscosity;
   k.c_grad = d.t.poly_6_gradient;
@@ -275,53 +275,54 @@ inline EdgeConsts edge_consts(const DevParams& d) {
   return k;
 }
 
-// The 32 edge inputs of edge (i <- j) in float32 (22 used, 10 zero). The
-// pair terms are the ones the forces stage sums (kernels/forces.cl:60-101):
-// pressure and viscosity accelerations in units of A, the colour-field
-// gradient term times h and its laplacian term times h^2.
-__device__ inline void edge_features(float4 xi, float4 xj, float4 vi, float4 vj, float rhoi,
-                                     float rhoj, float pi, float pj, const EdgeConsts& k,
-                                     float f[kFeat]) {
-  float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+// The 32 edge inputs of edge (i <- j) in float32 (22 used, 10 zero), from
+// the density stage's per-particle force inputs a = (x, p / rho^2) and
+// b = (v, m / rho) and the node inputs nu. The pair terms are the ones the
+// forces stage sums (kernels/forces.cl:60-101): pressure and viscosity
+// accelerations in units of A, the colour-field gradient term times h and
+// its laplacian term times h^2.
+__device__ inline void edge_features(float4 ai, float4 bi, float4 aj, float4 bj, float2 ni,
+                                     float2 nj, const EdgeConsts& k, float f[kFeat]) {
+  constexpr float inv_a = 1.f / kAccScale, inv_v = 1.f / kVelScale;
+  float dx = ai.x - aj.x, dy = ai.y - aj.y, dz = ai.z - aj.z;
   float d2 = dx * dx + dy * dy + dz * dz;
   float r = sqrtf(d2);
   float q = r * k.inv_h;
   float w = 1.f - q * q;
-  float2 ni = node_in(rhoi, pi, k.rho0, k.K), nj = node_in(rhoj, pj, k.rho0, k.K);
   f[0] = dx * k.inv_h, f[1] = dy * k.inv_h, f[2] = dz * k.inv_h;
-  f[3] = (vi.x - vj.x) / kVelScale, f[4] = (vi.y - vj.y) / kVelScale;
-  f[5] = (vi.z - vj.z) / kVelScale;
+  f[3] = (bi.x - bj.x) * inv_v, f[4] = (bi.y - bj.y) * inv_v, f[5] = (bi.z - bj.z) * inv_v;
   f[6] = q, f[7] = w * w * w;
   f[8] = ni.x, f[9] = ni.y, f[10] = nj.x, f[11] = nj.y;
   const float hr = k.h - r, t = k.h * k.h - d2;
-  const float ps = (pi / (rhoi * rhoi) + pj / (rhoj * rhoj)) * k.m;
-  const float g = r >= 1e-7f ? ps * k.c_spiky * hr * hr / r : 0.f;
-  f[12] = -g * dx / kAccScale, f[13] = -g * dy / kAccScale, f[14] = -g * dz / kAccScale;
-  const float wj = k.m / rhoj;
-  const float vw = k.mu / rhoi * wj * k.c_visc * hr / kAccScale;
-  f[15] = (vj.x - vi.x) * vw, f[16] = (vj.y - vi.y) * vw, f[17] = (vj.z - vi.z) * vw;
-  const float gw = wj * k.c_grad * t * t * k.h;
+  const float ps = (ai.w + aj.w) * k.m;
+  const float g = r >= 1e-7f ? ps * k.c_spiky * hr * hr * __builtin_amdgcn_rcpf(r) : 0.f;
+  f[12] = -g * dx * inv_a, f[13] = -g * dy * inv_a, f[14] = -g * dz * inv_a;
+  // mu / rho_i = (m / rho_i) (mu / m)
+  const float vw = bi.w * k.mu_over_m * bj.w * k.c_visc * hr * inv_a;
+  f[15] = (bj.x - bi.x) * vw, f[16] = (bj.y - bi.y) * vw, f[17] = (bj.z - bi.z) * vw;
+  const float gw = bj.w * k.c_grad * t * t * k.h;
   f[18] = gw * dx, f[19] = gw * dy, f[20] = gw * dz;
-  f[21] = wj * k.c_lap * t * (3.f * k.h * k.h - 7.f * d2) * k.h * k.h;
+  f[21] = bj.w * k.c_lap * t * (3.f * k.h * k.h - 7.f * d2) * k.h * k.h;
   for (int c = 22; c < kFeat; ++c) f[c] = 0.f;
 }
 
-// Edge inputs e (32 bf16 per edge) and node inputs nu.
+// nu_i = [rho_i / rho0 - 1, asinh(p_i / K) / 4] for every particle.
+__global__ void k_node_inputs(const float* __restrict__ rho, const float* __restrict__ prs,
+                              uint32_t n, float rho0, float K, float2* __restrict__ nu) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) nu[i] = node_in(rho[i], prs[i], rho0, K);
+}
+
+// Edge inputs e (32 bf16 per edge), after k_node_inputs.
 __global__ __launch_bounds__(256) void k_edge_inputs(
-    const float4* __restrict__ pos, const float4* __restrict__ vel,
-    const float* __restrict__ rho, const float* __restrict__ prs,
-    const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint32_t e, uint32_t n,
-    EdgeConsts kc, uint16_t* __restrict__ E1, float* __restrict__ nu) {
+    const float4* __restrict__ fa, const float4* __restrict__ fb, const float2* __restrict__ nu,
+    const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint32_t e,
+    EdgeConsts kc, uint16_t* __restrict__ E1) {
   uint32_t k = blockIdx.x * 256 + threadIdx.x;
-  if (k < n) {
-    float2 a = node_in(rho[k], prs[k], kc.rho0, kc.K);
-    nu[2 * k] = a.x;
-    nu[2 * k + 1] = a.y;
-  }
   if (k >= e) return;
   uint32_t i = dst[k], j = src[k];
   float f[kFeat];
-  edge_features(pos[i], pos[j], vel[i], vel[j], rho[i], rho[j], prs[i], prs[j], kc, f);
+  edge_features(fa[i], fb[i], fa[j], fb[j], nu[i], nu[j], kc, f);
   u32x4* o = (u32x4*)(E1 + (size_t)k * kFeat);
   for (int q = 0; q < kFeat / 8; ++q) {
     uint32_t packed[4];
@@ -357,13 +358,12 @@ __device__ inline u32x4 pack8(const float v[8]) {
 
 template <bool ROUND2>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_edge_fused(
-    const float4* __restrict__ pos, const float4* __restrict__ vel,
-    const float* __restrict__ rho, const float* __restrict__ prs,
+    const float4* __restrict__ fa, const float4* __restrict__ fb, const float2* __restrict__ nu,
     const uint4* __restrict__ nbr, const uint32_t* __restrict__ count, uint32_t n,
     EdgeConsts kc, const uint16_t* __restrict__ W1, const float* __restrict__ b1,
     const uint16_t* __restrict__ W2, const float* __restrict__ b2,
     const uint16_t* __restrict__ PA, const uint16_t* __restrict__ PB,
-    uint16_t* __restrict__ S, float* __restrict__ degc, float* __restrict__ nu) {
+    uint16_t* __restrict__ S, float* __restrict__ degc) {
   __shared__ u32x4 w1s[8 * 64];       // [ob][lane]: W1^T rows 16 ob + l % 16, k 8 g + t
   __shared__ u32x4 w2s[8 * 4 * 64];   // [ob][ks][lane]: W2^T in the permuted K order
   __shared__ float bs[2 * kWidth];
@@ -399,9 +399,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const uint32_t waves = gridDim.x * 4;
   for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += waves) {
     const uint32_t deg = min(count[i], kMaxEdges);
-    const float4 xi = pos[i], vi = vel[i];
-    const float rhoi = rho[i], pi = prs[i];
-    const float2 ni = node_in(rhoi, pi, kc.rho0, kc.K);
+    const float4 ai = fa[i], bi = fb[i];
+    const float2 ni = nu[i];
     float pa[8][4];
     for (int ob = 0; ob < 8; ++ob)
       for (int r = 0; r < 4; ++r) pa[ob][r] = bs[16 * ob + 4 * g + r];
@@ -427,7 +426,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       }
       // B operand of layer 1: features 8 g .. 8 g + 7 of this lane's edge
       float f[kFeat];
-      edge_features(xi, pos[j], vi, vel[j], rhoi, rho[j], pi, prs[j], kc, f);
+      edge_features(ai, bi, fa[j], fb[j], ni, nu[j], kc, f);
       float sel8[8];
       for (int t = 0; t < 8; ++t)
         sel8[t] = g == 0 ? f[t] : (g == 1 ? f[8 + t] : (g == 2 ? f[16 + t] : f[24 + t]));
@@ -492,11 +491,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                       ((uint32_t)to_bf16(acc[ob][3] * (1.f / kAggC)) << 16);
         *(uint2*)(S + (size_t)i * kWidth + 16 * ob + 4 * g) = make_uint2(lo, hi);
       }
-    if (lane == 0) {
-      degc[i] = (float)deg / kAggC;
-      nu[2 * i] = ni.x;
-      nu[2 * i + 1] = ni.y;
-    }
+    if (lane == 0) degc[i] = (float)deg / kAggC;
   }
 }
 
@@ -1105,9 +1100,10 @@ void launch_edge_inputs(clsph_nn* m, clsph_state* st) {
   const DevParams& d = st->host_dps[0];
   hipLaunchKernelGGL(k_graph, grid_for(n), dim3(256), 0, s, st->nbr, st->nbr_count, m->off, n,
                      m->src, m->dst, m->degc);
-  uint32_t cnt = e > n ? e : n;
-  hipLaunchKernelGGL(k_edge_inputs, grid_for(cnt), dim3(256), 0, s, st->pos_s, st->vel_s,
-                     st->rho_s, st->prs_s, m->src, m->dst, e, n, edge_consts(d), m->E1, m->nu);
+  hipLaunchKernelGGL(k_node_inputs, grid_for(n), dim3(256), 0, s, st->rho_s, st->prs_s, n,
+                     d.p.fluid_density, d.p.K, (float2*)m->nu);
+  hipLaunchKernelGGL(k_edge_inputs, grid_for(e), dim3(256), 0, s, st->fa, st->fb,
+                     (const float2*)m->nu, m->src, m->dst, e, edge_consts(d), m->E1);
 }
 
 static inline const float* P(const clsph_nn* m, int id) { return m->params + param_offset(id); }
@@ -1203,10 +1199,12 @@ void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float gx, float gy
   if (blocks * 4 > n) blocks = (n + 3) / 4;
   if (blocks == 0) blocks = 1;
   if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL((k_edge_fused<false>), dim3(blocks), dim3(256), 0, s, st->pos_s, st->vel_s,
-                     st->rho_s, st->prs_s, st->nbr, st->nbr_count, n, edge_consts(d),
+  hipLaunchKernelGGL(k_node_inputs, grid_for(n), dim3(256), 0, s, st->rho_s, st->prs_s, n,
+                     d.p.fluid_density, d.p.K, (float2*)m->nu);
+  hipLaunchKernelGGL((k_edge_fused<false>), dim3(blocks), dim3(256), 0, s, st->fa, st->fb,
+                     (const float2*)m->nu, st->nbr, st->nbr_count, n, edge_consts(d),
                      W(m, E1_W1), P(m, E1_B1), W(m, E1_W2), P(m, E1_B2),
-                     (const uint16_t*)nullptr, (const uint16_t*)nullptr, m->S[0], m->degc, m->nu);
+                     (const uint16_t*)nullptr, (const uint16_t*)nullptr, m->S[0], m->degc);
   if (ev) (void)hipEventRecord(ev[1], s);
   // node part of round 1 (fused)
   constexpr size_t kLdsNode1 = (size_t)(8 * 4 + 8 * 5 + 8 * 4 + 8 * 4) * 64 * 16 + 4 * kWidth * 4;
@@ -1232,10 +1230,10 @@ void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float gx, float gy
   hipLaunchKernelGGL(k_node_terms, dim3(nblocks), dim3(512), kLdsTerms, s, m->Hb, n, m->wb, m->PA,
                      m->PB);
   if (ev) (void)hipEventRecord(ev[2], s);
-  hipLaunchKernelGGL((k_edge_fused<true>), dim3(blocks), dim3(256), 0, s, st->pos_s, st->vel_s,
-                     st->rho_s, st->prs_s, st->nbr, st->nbr_count, n, edge_consts(d),
+  hipLaunchKernelGGL((k_edge_fused<true>), dim3(blocks), dim3(256), 0, s, st->fa, st->fb,
+                     (const float2*)m->nu, st->nbr, st->nbr_count, n, edge_consts(d),
                      W(m, E2_W1), P(m, E2_B1), W(m, E2_W2), P(m, E2_B2), m->PA, m->PB,
-                     m->S[1], m->degc, m->nu);
+                     m->S[1], m->degc);
   if (ev) (void)hipEventRecord(ev[3], s);
   hipLaunchKernelGGL(k_node2, dim3(nblocks), dim3(512), kLdsNode2, s, m->S[1], m->Hb, m->degc, n,
                      m->wb, m->params, m->out, acc, gx, gy, gz);

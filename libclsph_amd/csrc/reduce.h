@@ -14,8 +14,7 @@ __device__ inline float wave_max(float v) {
   return v;
 }
 
-// Work-group min/max of (x, y, z) into partial[L * 6 + k], L the logical
-// work-group of particle_of (xcd_block); NaN
+// Work-group min/max of (x, y, z) into partial[blockIdx.x * 6 + k]; NaN
 // coordinates are skipped, like the reference's `<` / `>` tests
 // (sph_simulation.cpp:207-217). Every thread of the work-group must call it.
 __device__ inline void block_aabb_partial(float4 p, bool valid, float* __restrict__ partial) {
@@ -39,7 +38,7 @@ __device__ inline void block_aabb_partial(float4 p, bool valid, float* __restric
     int nw = (blockDim.x + 63) >> 6;
     float r = red[k][0];
     for (int q = 1; q < nw; ++q) r = (k < 3) ? fminf(r, red[k][q]) : fmaxf(r, red[k][q]);
-    partial[xcd_block(blockIdx.x, gridDim.x) * 6 + k] = r;
+    partial[blockIdx.x * 6 + k] = r;
   }
 }
 
