@@ -78,11 +78,12 @@ def init_params(seed=0):
 
 def slab_variant(seed, spacing=SPACING_1M, depth_sites=12):
     """A dam-break variant at the spacing of the 1,048,576-particle dam
-    break: block b x 2b x (depth_sites spacings) with b ~ U(0.3, 0.5) m, in a
-    2.5 b x 2.5 b box of the same depth; jitter ~ U(0, 0.02) spacings and
-    dynamic viscosity ~ U(1e-3, 1e-2). Twelve sites in depth leave the middle
-    layers with complete neighbourhoods (radius h = 2 spacings) under the
-    full column height."""
+    break: block b x 2b x d (d = depth_sites spacings) with b ~ U(0.3, 0.5) m
+    in the corner of a 2.5 b x 2.5 b x 2.5 d box, as the 1M block fills 40 %
+    of its box along x and z (walls behind it, free faces in front); jitter
+    ~ U(0, 0.02) spacings and dynamic viscosity ~ U(1e-3, 1e-2). Twelve sites
+    in depth leave the middle layers with complete neighbourhoods (radius
+    h = 2 spacings) under the full column height."""
     rng = np.random.default_rng(seed)
     b = float(rng.uniform(0.3, 0.5))
     s = np.float32(spacing)
@@ -99,7 +100,7 @@ def slab_variant(seed, spacing=SPACING_1M, depth_sites=12):
                               jitter, int(seed) * 7919 + 11)
     params = scenes.fluid_params(n, h, mass, viscosity=visc)
     terms = native.kernel_terms(float(h))
-    normals, verts, idx = native.make_box((0.0, 0.0, 0.0), (2.5 * b, 2.5 * b, depth))
+    normals, verts, idx = native.make_box((0.0, 0.0, 0.0), (2.5 * b, 2.5 * b, 2.5 * depth))
     return scenes.Scene(parts, params, terms, normals, verts, idx)
 
 
