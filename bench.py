@@ -105,7 +105,8 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--particles", type=int, default=None)
-    ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="substeps of the per-stage timing (0: the timed substeps)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--track-ids", action="store_true",
@@ -150,6 +151,11 @@ def main():
 
     if args.warmup:
         st.step(args.warmup)
+    # the state at the start of the timed region, to time its stages below
+    start = native.State(ctx, n)
+    start.set_params(scene.params, scene.terms)
+    start.scene(scene.normals, scene.vertices, scene.indices)
+    start.copy_from(st)
     ctx.sync()
     t0 = time.perf_counter()
     ms = st.time_steps(args.steps)  # HIP events around K graph replays
@@ -160,8 +166,10 @@ def main():
     print(f"# timed {args.steps} substeps: {ms:.2f} ms (events), {wall * 1e3:.2f} ms (wall)",
           file=sys.stderr, flush=True)
 
-    # Per-stage durations on the same state (HIP events between stages).
-    prof = st.profile_steps(args.profile_steps)
+    # Per-stage durations over the same substeps: the timed region replayed
+    # stage by stage from its starting state, HIP events between stages.
+    prof = start.profile_steps(args.steps if args.profile_steps <= 0 else args.profile_steps)
+    start.close()
     grid = st.grid()
     cells = int(grid["grid_cell_count"])
     nbytes = stage_bytes(n, cells)
@@ -324,7 +332,7 @@ def bench_batched(args):
     ctx.sync()
     ms = b.time_steps(steps)
     ms_step = ms / steps
-    prof = b.profile_steps(args.profile_steps)
+    prof = b.profile_steps(args.profile_steps if args.profile_steps > 0 else 50)
     finite = all(bool(np.isfinite(b.download_scene(i)["position"][:, :3]).all())
                  for i in range(0, 64, 7))
     b.close()
