@@ -177,13 +177,15 @@ int clsph_profile_steps(clsph_state* st, uint32_t nsteps, float* stage_ms);
 
 /* ---- learned force surrogate ----
  * No counterpart in the reference, whose only acceleration model is the
- * `forces` kernel (kernels/sph.cl:42-62). A two-round message-passing
+ * `forces` kernel (kernels/sph.cl:42-62). A one-round message-passing
  * network over the neighbour graph of the density stage (r < h, at most 128
- * neighbours per particle) predicts each particle's acceleration; its
- * products run on bf16 matrix cores with float32 accumulation. Parameters
- * are one float32 vector of clsph_nn_param_count() values (row-major
- * inputs x outputs matrices and biases in a fixed order). Single-scene
- * states only. */
+ * neighbours per particle) corrects the forces stage's acceleration a0 so
+ * that it stands for the mean acceleration over a macro step of 4 substeps:
+ * a = a0 + 2 out[0:3] (m/s^2). Its products run on bf16 matrix cores with
+ * float32 accumulation; with all parameters zero it returns a0 unchanged.
+ * Parameters are one float32 vector of clsph_nn_param_count() values
+ * (row-major inputs x outputs matrices and biases in a fixed order).
+ * Single-scene states only. */
 typedef struct clsph_nn clsph_nn;
 uint32_t clsph_nn_param_count(void);
 /* Workspace for states of up to n_cap particles and graphs of up to e_cap
@@ -193,33 +195,38 @@ int clsph_nn_free(clsph_nn* nn);
 /* Sets the parameters (and clears the optimiser moments) / reads them. */
 int clsph_nn_set_params(clsph_nn* nn, const float* params, uint32_t count);
 int clsph_nn_get_params(clsph_nn* nn, float* params, uint32_t count);
-/* Evaluates the network on a state whose last stage was density: builds the
- * graph, writes the accelerations where the forces stage would (the
- * following advection uses them) and the edge count to *edges (optional). */
+/* Training evaluation on a state whose last stage was forces: builds the
+ * graph, stores every intermediate the backward pass needs and the output
+ * (clsph_nn_read_output), writes the edge count to *edges (optional). The
+ * state is not changed. */
 int clsph_nn_forward(clsph_nn* nn, clsph_state* st, uint32_t* edges);
-/* The same evaluation with the fused inference kernels: no per-edge
- * storage (any edge count), no host synchronisation, no gradients. */
+/* Inference on a state whose last stage was forces, with one fused kernel
+ * (no per-edge storage, any edge count, no host synchronisation): adds the
+ * correction to the forces stage's accelerations, which the following
+ * advection uses, and stores the output. Once per forces stage. */
 int clsph_nn_infer(clsph_nn* nn, clsph_state* st);
-/* Runs clsph_nn_infer `reps` times with HIP events on the context stream;
- * ms[0..3] = average milliseconds of edge round 1, the node part of round 1,
- * edge round 2 and the node part of round 2; *edges (optional) = the edges
- * evaluated per pass. */
+/* Times the inference `reps` times with HIP events on the context stream,
+ * on a copy of the accelerations (the state is not changed): ms[0] =
+ * average milliseconds per evaluation; *edges (optional) = the edges
+ * evaluated per pass. The last stage must be forces. */
 int clsph_nn_profile(clsph_nn* nn, clsph_state* st, uint32_t reps, float* ms, uint64_t* edges);
-/* The graph of the last evaluation (dst-major CSR: off[n + 1], src[e]) and
- * the raw network output (n x 32 floats; columns 0..2 are (a - g) / 10). */
+/* The graph of the last training evaluation (dst-major CSR: off[n + 1],
+ * src[e]) and the raw output of the last evaluation (n x 32 floats; columns
+ * 0..2 are (a - a0) / 2). */
 int clsph_nn_read_graph(clsph_nn* nn, uint32_t* off, uint32_t* src, uint32_t n, uint32_t e);
 int clsph_nn_read_output(clsph_nn* nn, float* out, uint32_t n);
 /* Training target (n x 3, in the output's units) from the host, or from two
- * states: `learner` sorted for the current substep and `teacher` started as
- * a copy of the learner's particles with identities tracked, then advanced
- * by the plain method over `dt` seconds. */
+ * states: `learner` after its forces stage (a0) and `teacher` started as a
+ * copy of the learner's particles with identities tracked, then advanced by
+ * the plain method over `dt` seconds: ((v_teacher - v_learner) / dt - a0) / 2. */
 int clsph_nn_set_target(clsph_nn* nn, const float* target, uint32_t n);
 int clsph_nn_read_target(clsph_nn* nn, float* target, uint32_t n);
 int clsph_nn_target_from(clsph_nn* nn, clsph_state* learner, clsph_state* teacher, float dt);
-/* Target = the mean over `nsub` plain substeps of each particle's
- * forces-stage acceleration: `teacher` (a copy of the learner's particles,
- * identities tracked) is advanced by nsub substeps stage by stage. Collision
- * impulses, which advection applies, are left out. */
+/* Target = (mean over `nsub` plain substeps of each particle's forces-stage
+ * acceleration - a0) / 2, `learner` after its forces stage: `teacher` (a
+ * copy of the learner's particles, identities tracked) is advanced by nsub
+ * substeps stage by stage. Collision impulses, which advection applies, are
+ * left out. */
 int clsph_nn_target_mean_acc(clsph_nn* nn, clsph_state* learner, clsph_state* teacher,
                              uint32_t nsub);
 /* Mean squared error of the last evaluation against the target and its
@@ -228,8 +235,8 @@ int clsph_nn_target_mean_acc(clsph_nn* nn, clsph_state* learner, clsph_state* te
 int clsph_nn_backward(clsph_nn* nn, float* loss);
 int clsph_nn_read_grad(clsph_nn* nn, float* grad, uint32_t count);
 int clsph_nn_adam(clsph_nn* nn, float lr, float beta1, float beta2, float eps);
-/* `nsteps` substeps with the network (fused inference kernels) in place of
- * the forces stage, at the state's time_delta. */
+/* `nsteps` substeps at the state's time_delta, each the plain substep with
+ * the network's correction added to the forces stage's accelerations. */
 int clsph_learned_step(clsph_state* st, clsph_nn* nn, uint32_t nsteps);
 /* Density difference of the same particles in two states after both ran
  * their density stage (identities tracked in both): max and mean of

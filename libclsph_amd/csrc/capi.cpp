@@ -275,6 +275,7 @@ int run_stage(clsph_state* st, int stage) {
       return fail(st->ctx, CLSPH_EINVAL, "unknown stage " + std::to_string(stage));
   }
   st->stage = stage;
+  st->acc_learned = false;
   return check_launch(st->ctx);
 }
 

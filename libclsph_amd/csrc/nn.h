@@ -1,19 +1,20 @@
-// Learned force surrogate: a two-round message-passing network over the
-// particles' neighbour graph (r < h), evaluated and trained with bfloat16
-// matrix-core products accumulated in float32.
+// Learned force surrogate: a one-round message-passing network over the
+// particles' neighbour graph (r < h) that corrects the forces stage's
+// acceleration so that it stands for the mean acceleration over a macro step
+// of 4 substeps. Evaluated and trained with bfloat16 matrix-core products
+// accumulated in float32.
 //
 // Network (per particle i, neighbours j; all products bf16 x bf16 -> f32):
 //   nu_i  = [rho_i / rho0 - 1, asinh(p_i / K) / 4]
 //   e_ij  = [(x_i - x_j) / h, (v_i - v_j) / V, q, (1 - q^2)^3, nu_i, nu_j,
 //            pressure and viscosity pair accelerations / A, colour-field
 //            gradient term * h and laplacian term * h^2, 0 x 10]
-//   round 1: s1 = silu(e W1 + b1), s2 = silu(s1 W2 + b2),
-//            M1_i = bf16(sum_j s2_ij / C) W3 + (deg_i / C) b3
-//            H_i  = 3-layer SiLU MLP of [M1_i, nu_i, 0 x 14]      (144 -> 128)
-//   round 2: z1 = e W1' + b1' + bf16(H_i Wa) + bf16(H_j Wb), then as round 1
-//            out_i = 3-layer SiLU MLP of [H_i, M2_i]                (256 -> 32)
-//   acceleration a_i = g + A out_i[0:3]
-// The third edge layer is linear and is applied after the neighbour sum.
+//   edges: S_i = bf16(sum_j relu(e_ij W1 + b1) / C)                (32 wide)
+//   node:  u_i = [S_i, nu_i, (a0_i - g) / A, deg_i / C, 0 x 26]     (64 wide)
+//          out_i = 3-layer MLP of u_i, SiLU between layers (64 -> 64 -> 64 -> 32)
+//   acceleration a_i = a0_i + R out_i[0:3]
+// a0 is the forces stage's acceleration of the same substep; a zero network
+// leaves it unchanged.
 #pragma once
 
 #include <stdint.h>
@@ -23,30 +24,22 @@
 namespace clsph {
 namespace nn {
 
-constexpr int kWidth = 128;
-constexpr int kFeat = 32;
-constexpr int kIn1 = 144;
-constexpr int kIn2 = 256;
-constexpr int kOut = 32;
+constexpr int kEdgeW = 32;   // edge layer width
+constexpr int kNodeW = 64;   // node MLP width
+constexpr int kFeat = 32;    // edge inputs (22 used)
+constexpr int kFeatUsed = 22;
+constexpr int kIn = 64;      // node MLP input (38 used)
+constexpr int kOut = 32;     // network output (3 used)
 constexpr float kAggC = 32.f;
 constexpr float kVelScale = 1.f;
-constexpr float kAccScale = 10.f;
+constexpr float kAccScale = 10.f;  // A
+constexpr float kResScale = 2.f;   // R
 constexpr uint32_t kMaxEdges = 128;  // neighbours per particle the network sees
 
 // Flat parameter vector: row-major (inputs x outputs) matrices and biases.
-enum Param {
-  E1_W1, E1_B1, E1_W2, E1_B2, E1_W3, E1_B3,
-  N1_W1, N1_B1, N1_W2, N1_B2, N1_W3, N1_B3,
-  E2_W1, E2_B1, E2_WA, E2_WB, E2_W2, E2_B2, E2_W3, E2_B3,
-  N2_W1, N2_B1, N2_W2, N2_B2, N2_W3, N2_B3,
-  kNumParams
-};
+enum Param { E_W1, E_B1, N_W1, N_B1, N_W2, N_B2, N_W3, N_B3, kNumParams };
 constexpr uint32_t kParamSize[kNumParams] = {
-    kFeat * kWidth, kWidth, kWidth * kWidth, kWidth, kWidth * kWidth, kWidth,
-    kIn1 * kWidth, kWidth, kWidth * kWidth, kWidth, kWidth * kWidth, kWidth,
-    kFeat * kWidth, kWidth, kWidth * kWidth, kWidth * kWidth, kWidth * kWidth, kWidth,
-    kWidth * kWidth, kWidth,
-    kIn2 * kWidth, kWidth, kWidth * kWidth, kWidth, kWidth * kOut, kOut};
+    kFeat * kEdgeW, kEdgeW, kIn * kNodeW, kNodeW, kNodeW * kNodeW, kNodeW, kNodeW * kOut, kOut};
 
 constexpr uint32_t param_offset(int p) {
   uint32_t o = 0;
@@ -78,45 +71,28 @@ struct clsph_nn {
   uint32_t* dst = nullptr;      // e_cap
   uint32_t* scan_part = nullptr;
 
-  // forward intermediates
-  uint16_t* E1 = nullptr;       // e x 16 edge inputs
+  // forward intermediates of the training path
+  uint16_t* E1 = nullptr;       // e x 32 edge inputs
   float* nu = nullptr;          // n x 2
   float* degc = nullptr;        // deg / C
-  float* z1[2] = {};            // e x 128, per round
-  uint16_t* s1[2] = {};
-  float* z2[2] = {};
-  uint16_t* S[2] = {};          // n x 128: bf16(sum_j s2 / C)
-  float* tmp = nullptr;         // n x 256 float scratch
-  uint16_t* u1 = nullptr;       // n x 144
-  float* p1 = nullptr;
+  float* z = nullptr;           // e x 32 edge pre-activations
+  uint16_t* S = nullptr;        // n x 32: bf16(sum_j relu(z) / C)
+  uint16_t* u = nullptr;        // n x 64 node inputs
+  float* p1 = nullptr;          // n x 64
   uint16_t* y1 = nullptr;
   float* p2 = nullptr;
   uint16_t* y2 = nullptr;
-  uint16_t* Hb = nullptr;
-  uint16_t* PA = nullptr;
-  uint16_t* PB = nullptr;
-  uint16_t* u2 = nullptr;       // n x 256
-  float* q1 = nullptr;
-  uint16_t* w1 = nullptr;
-  float* q2 = nullptr;
-  uint16_t* w2 = nullptr;
   float* out = nullptr;         // n x 32
 
   // training
   float* target = nullptr;      // n x 3
   float* loss = nullptr;        // 1
   float* dout = nullptr;        // n x 32
-  float* dY = nullptr;          // n x 256
-  uint16_t* dpb = nullptr;      // n x 256 bf16 scratch
-  float* dS = nullptr;          // n x 128
-  uint16_t* dz2 = nullptr;      // e x 128
-  float* ds1 = nullptr;         // e x 128
-  uint16_t* dz1 = nullptr;      // e x 128
-  float* dPA = nullptr;
-  float* dPB = nullptr;
-  float* dH = nullptr;
-  uint16_t* dPA_b = nullptr;
-  uint16_t* dPB_b = nullptr;
+  uint16_t* dout_b = nullptr;   // n x 32
+  float* dY = nullptr;          // n x 64
+  uint16_t* dpb = nullptr;      // n x 64
+  uint16_t* dz = nullptr;       // e x 32
+  float* sum = nullptr;         // n x 3 acceleration sums of the mean target
   uint32_t* loc = nullptr;      // n: inverse of a state's ids
   uint32_t* inv = nullptr;      // n: inverse of a sort permutation
   float* cmp = nullptr;         // 2: max and sum of |rho_a - rho_b|
@@ -128,29 +104,29 @@ namespace nn {
 // Graph + edge inputs from a state whose density stage has run.
 void launch_graph(clsph_nn* m, clsph_state* s);
 void launch_edge_inputs(clsph_nn* m, clsph_state* s);
-// Forward pass of the toolbox kernels; `acc` (float4 per particle) receives
-// the accelerations when non-null.
-void forward(clsph_nn* m, const DevParams* dp_host, float4* acc, float gx, float gy, float gz);
-// Inference forward with the fused edge kernels (reads the state's neighbour
-// lists directly; no per-edge storage, no edge-count readback).
-// With `ev` (5 events) the stream gets events before edge round 1, after it,
-// after the node part of round 1, after edge round 2 and at the end.
-void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float gx, float gy, float gz,
-                   hipEvent_t* ev = nullptr);
+// Training forward (graph built): reads the forces stage's acceleration from
+// `acc0` and leaves the output in m->out; the state is not changed.
+void forward(clsph_nn* m, const float4* acc0, float gx, float gy, float gz);
+// Inference forward, one fused kernel over the state's neighbour lists (no
+// per-edge storage, no edge-count readback): acc[i] += R out_i[0:3] where
+// acc holds the forces stage's acceleration. With `out` non-null the full
+// output is written there as well. With `ev` (2 events) the stream gets
+// events before and after the kernel.
+void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float* out, float gx, float gy,
+                   float gz, hipEvent_t* ev = nullptr);
 // Loss against m->target and gradients into m->grad (zeroed first).
 void backward(clsph_nn* m);
 void adam(clsph_nn* m, float lr, float beta1, float beta2, float eps);
 void cast_params(clsph_nn* m);
-// target[i] = ((vh_T[loc[perm_S[i]]] - vh_S[perm_S[i]]) / dt - g) / A
-void launch_target(clsph_nn* m, clsph_state* learner, clsph_state* teacher, float dt,
-                   float gx, float gy, float gz);
+// target[i] = ((vh_T[loc[perm_S[i]]] - vh_S[perm_S[i]]) / dt - a0_i) / R,
+// a0 the learner's forces-stage acceleration at sorted i.
+void launch_target(clsph_nn* m, clsph_state* learner, clsph_state* teacher, float dt);
 // Advances `teacher` (a copy of the learner's particles with identities
 // reset) by nsub plain substeps stage by stage, summing each particle's
-// forces-stage acceleration by identity; target[i] = (mean - g) / A for the
-// learner's sorted particle i. Collision impulses (applied by advection)
-// are not part of the target.
-void launch_target_mean(clsph_nn* m, clsph_state* learner, clsph_state* teacher, uint32_t nsub,
-                        float gx, float gy, float gz);
+// forces-stage acceleration by identity; target[i] = (mean - a0_i) / R for
+// the learner's sorted particle i, a0 its forces-stage acceleration.
+// Collision impulses (applied by advection) are not part of the target.
+void launch_target_mean(clsph_nn* m, clsph_state* learner, clsph_state* teacher, uint32_t nsub);
 // Per-particle density difference of two states holding the same particles
 // (matched through their identities); res (host, read after a
 // synchronisation) = {max |rho_a - rho_b|, sum |rho_a - rho_b|}.

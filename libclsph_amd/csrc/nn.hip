@@ -318,202 +318,71 @@ __global__ __launch_bounds__(256) void k_edge_inputs(
     const float4* __restrict__ fa, const float4* __restrict__ fb, const float2* __restrict__ nu,
     const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint32_t e,
     EdgeConsts kc, uint16_t* __restrict__ E1) {
-  uint32_t k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= e) return;
-  uint32_t i = dst[k], j = src[k];
-  float f[kFeat];
-  edge_features(fa[i], fb[i], fa[j], fb[j], nu[i], nu[j], kc, f);
-  u32x4* o = (u32x4*)(E1 + (size_t)k * kFeat);
-  for (int q = 0; q < kFeat / 8; ++q) {
-    uint32_t packed[4];
-    for (int t = 0; t < 4; ++t)
-      packed[t] = (uint32_t)to_bf16(f[8 * q + 2 * t]) |
-                  ((uint32_t)to_bf16(f[8 * q + 2 * t + 1]) << 16);
-    o[q] = u32x4{packed[0], packed[1], packed[2], packed[3]};
+  for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < e; k += gridDim.x * 256) {
+    uint32_t i = dst[k], j = src[k];
+    float f[kFeat];
+    edge_features(fa[i], fb[i], fa[j], fb[j], nu[i], nu[j], kc, f);
+    u32x4* o = (u32x4*)(E1 + (size_t)k * kFeat);
+    for (int q = 0; q < kFeat / 8; ++q) {
+      uint32_t packed[4];
+      for (int t = 0; t < 4; ++t)
+        packed[t] = (uint32_t)to_bf16(f[8 * q + 2 * t]) |
+                    ((uint32_t)to_bf16(f[8 * q + 2 * t + 1]) << 16);
+      o[q] = u32x4{packed[0], packed[1], packed[2], packed[3]};
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Fused edge round for inference: one wave per particle i walks its
-// neighbour list (the density kernel's, at most kMaxEdges) in tiles of 16
-// edges and computes S_i = bf16(sum_j silu(silu(e_ij W1 + b1 [+ PA_i + PB_j])
-// W2 + b2) / C) without storing per-edge values.
+// Fused inference: one kernel walks the density stage's neighbour lists and
+// evaluates the whole network.
 //
-// Both layers run transposed on 16x16x32 bf16 matrix cores: z^T = W^T e^T,
-// so the output's columns are edges and a lane holds, for edge l % 16, the
-// features 16 ob + 4 (l / 16) + r of output block ob (r = 0..3). Layer 2's
-// B operand (lane: 8 consecutive K slots of its edge) is taken straight from
-// those registers by permuting the K order: slot t of k-step ks and lane
-// group g = l / 16 is feature 32 ks + 16 (t / 4) + 4 g + t % 4, and the
-// weights are staged in LDS in that order. Tiles accumulate per lane; the
-// 16 edge lanes are summed once per particle.
+// A wave owns tiles of 16 consecutive sorted particles; lane l serves
+// particle c = l % 16 of the tile in lane group g = l / 16. The edge layer
+// runs transposed on 16x16x32 bf16 matrix cores, z^T = W1^T e^T, with the
+// tile's particles as the 16 columns: at step s, column c holds particle c's
+// s-th neighbour edge, so each lane's accumulator sums its own particle's
+// edges in list order and the neighbour sum needs no cross-lane reduction.
+// A lane holds z for hidden units 16 ob + 4 g + r (r = 0..3) of its column.
+//
+// Edge inputs are computed once per edge. In each round of 4 steps, lane
+// (g, c) computes the inputs of particle c's edge 4 k + g and stores inputs
+// 0..23 as bf16 in the wave's LDS block; the B operand of step 4 k + s for
+// lane (g, c) is inputs 8 g .. 8 g + 7 of edge 4 k + s of particle c (lane
+// group 3 reads group 2's inputs, which meet the zero rows 24..31 of W1^T).
+// Per particle column the block holds 4 edges x 48 bytes plus 16 bytes of
+// padding: 208 bytes = 52 dwords, so the 16 columns of one read start on 16
+// distinct multiples of 4 banks.
+//
+// The node MLP follows in registers: the neighbour sums are the B operand of
+// node layer 1 in the permuted K order (slot t of lane group g is feature
+// 16 (t / 4) + 4 g + t % 4 of a 32-wide block), node inputs 32..63 come in
+// the natural order (slot t of group g is feature 32 + 8 g + t), and each
+// later layer takes its B operand from the previous layer's registers in the
+// permuted order. Weights are staged once per work-group in LDS in the
+// matching orders.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ inline u32x4 pack8(const float v[8]) {
+constexpr int kTileP = 16;                 // particles per tile (MFMA columns)
+constexpr int kSurWaves = 8;               // waves per work-group
+constexpr int kColBytes = 4 * 48 + 16;     // LDS bytes per particle column
+constexpr int kWaveLds = kTileP * kColBytes;
+
+__device__ __attribute__((always_inline)) inline u32x4 pack8(const float v[8]) {
   u32x4 r;
   for (int t = 0; t < 4; ++t)
     r[t] = (uint32_t)to_bf16(v[2 * t]) | ((uint32_t)to_bf16(v[2 * t + 1]) << 16);
   return r;
 }
 
-template <bool ROUND2>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_edge_fused(
-    const float4* __restrict__ fa, const float4* __restrict__ fb, const float2* __restrict__ nu,
-    const uint4* __restrict__ nbr, const uint32_t* __restrict__ count, uint32_t n,
-    EdgeConsts kc, const uint16_t* __restrict__ W1, const float* __restrict__ b1,
-    const uint16_t* __restrict__ W2, const float* __restrict__ b2,
-    const uint16_t* __restrict__ PA, const uint16_t* __restrict__ PB,
-    uint16_t* __restrict__ S, float* __restrict__ degc) {
-  __shared__ u32x4 w1s[8 * 64];       // [ob][lane]: W1^T rows 16 ob + l % 16, k 8 g + t
-  __shared__ u32x4 w2s[8 * 4 * 64];   // [ob][ks][lane]: W2^T in the permuted K order
-  __shared__ float bs[2 * kWidth];
-  for (int idx = threadIdx.x; idx < 8 * 64; idx += 256) {
-    int ob = idx >> 6, l = idx & 63, g = l >> 4, row = 16 * ob + (l & 15);
-    uint32_t w[4];
-    for (int t = 0; t < 4; ++t) {
-      int k0 = 8 * g + 2 * t;
-      uint32_t lo = k0 < kFeat ? W1[k0 * kWidth + row] : 0u;
-      uint32_t hi = k0 + 1 < kFeat ? W1[(k0 + 1) * kWidth + row] : 0u;
-      w[t] = lo | (hi << 16);
-    }
-    w1s[idx] = u32x4{w[0], w[1], w[2], w[3]};
-  }
-  for (int idx = threadIdx.x; idx < 8 * 4 * 64; idx += 256) {
-    int ob = idx >> 8, ks = (idx >> 6) & 3, l = idx & 63, g = l >> 4, row = 16 * ob + (l & 15);
-    uint32_t w[4];
-    for (int t = 0; t < 4; ++t) {
-      int ta = 2 * t, tb = 2 * t + 1;
-      int ka = 32 * ks + 16 * (ta >> 2) + 4 * g + (ta & 3);
-      int kb = 32 * ks + 16 * (tb >> 2) + 4 * g + (tb & 3);
-      w[t] = (uint32_t)W2[ka * kWidth + row] | ((uint32_t)W2[kb * kWidth + row] << 16);
-    }
-    w2s[idx] = u32x4{w[0], w[1], w[2], w[3]};
-  }
-  for (int idx = threadIdx.x; idx < kWidth; idx += 256) {
-    bs[idx] = b1[idx];
-    bs[kWidth + idx] = b2[idx];
-  }
-  __syncthreads();
-
-  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
-  const uint32_t waves = gridDim.x * 4;
-  for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += waves) {
-    const uint32_t deg = min(count[i], kMaxEdges);
-    const float4 ai = fa[i], bi = fb[i];
-    const float2 ni = nu[i];
-    float pa[8][4];
-    for (int ob = 0; ob < 8; ++ob)
-      for (int r = 0; r < 4; ++r) pa[ob][r] = bs[16 * ob + 4 * g + r];
-    if (ROUND2) {
-      for (int ob = 0; ob < 8; ++ob) {
-        uint2 v = *(const uint2*)(PA + (size_t)i * kWidth + 16 * ob + 4 * g);
-        pa[ob][0] += from_bf16((uint16_t)(v.x & 0xFFFFu));
-        pa[ob][1] += from_bf16((uint16_t)(v.x >> 16));
-        pa[ob][2] += from_bf16((uint16_t)(v.y & 0xFFFFu));
-        pa[ob][3] += from_bf16((uint16_t)(v.y >> 16));
-      }
-    }
-    f32x4 acc[8];
-    for (int ob = 0; ob < 8; ++ob) acc[ob] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (uint32_t q0 = 0; q0 < deg; q0 += 16) {
-      const uint32_t q = q0 + c;
-      const bool valid = q < deg;
-      uint32_t j = i;
-      if (valid) {
-        uint4 js = nbr[(size_t)(q >> 2) * n + i];
-        uint32_t sel = q & 3u;
-        j = sel == 0 ? js.x : sel == 1 ? js.y : sel == 2 ? js.z : js.w;
-      }
-      // B operand of layer 1: features 8 g .. 8 g + 7 of this lane's edge
-      float f[kFeat];
-      edge_features(ai, bi, fa[j], fb[j], ni, nu[j], kc, f);
-      float sel8[8];
-      for (int t = 0; t < 8; ++t)
-        sel8[t] = g == 0 ? f[t] : (g == 1 ? f[8 + t] : (g == 2 ? f[16 + t] : f[24 + t]));
-      const bf16x8 eb = __builtin_bit_cast(bf16x8, pack8(sel8));
-      // layer 1 (+ bias, node terms), SiLU, packed into layer 2's B operands
-      float pb[8][4];
-      if (ROUND2) {
-        for (int ob = 0; ob < 8; ++ob) {
-          uint2 v = *(const uint2*)(PB + (size_t)j * kWidth + 16 * ob + 4 * g);
-          pb[ob][0] = from_bf16((uint16_t)(v.x & 0xFFFFu));
-          pb[ob][1] = from_bf16((uint16_t)(v.x >> 16));
-          pb[ob][2] = from_bf16((uint16_t)(v.y & 0xFFFFu));
-          pb[ob][3] = from_bf16((uint16_t)(v.y >> 16));
-        }
-      }
-      float s1[8][4];
-      for (int ob = 0; ob < 8; ++ob) {
-        f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-            __builtin_bit_cast(bf16x8, w1s[ob * 64 + lane]), eb, f32x4{0.f, 0.f, 0.f, 0.f}, 0,
-            0, 0);
-        for (int r = 0; r < 4; ++r) {
-          float v = z[r] + pa[ob][r];
-          if (ROUND2) v += pb[ob][r];
-          s1[ob][r] = silu(v);
-        }
-      }
-      bf16x8 bop[4];
-      for (int ks = 0; ks < 4; ++ks) {
-        float v8[8];
-        for (int t = 0; t < 8; ++t) v8[t] = s1[2 * ks + (t >> 2)][t & 3];
-        bop[ks] = __builtin_bit_cast(bf16x8, pack8(v8));
-      }
-      // layer 2, SiLU, masked accumulation over the tile's edges
-      for (int ob = 0; ob < 8; ++ob) {
-        f32x4 y = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int ks = 0; ks < 4; ++ks)
-          y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-              __builtin_bit_cast(bf16x8, w2s[(ob * 4 + ks) * 64 + lane]), bop[ks], y, 0, 0, 0);
-        for (int r = 0; r < 4; ++r) {
-          float v = silu(y[r] + bs[kWidth + 16 * ob + 4 * g + r]);
-          acc[ob][r] += valid ? v : 0.f;
-        }
-        __builtin_amdgcn_sched_barrier(0);  // one output block's reads at a time
-      }
-    }
-    // sum over the 16 edge lanes of each group
-    for (int ob = 0; ob < 8; ++ob)
-      for (int r = 0; r < 4; ++r) {
-        float v = acc[ob][r];
-        v += __shfl_xor(v, 1, 64);
-        v += __shfl_xor(v, 2, 64);
-        v += __shfl_xor(v, 4, 64);
-        v += __shfl_xor(v, 8, 64);
-        acc[ob][r] = v;
-      }
-    // lane c < 8 of group g writes features 16 c + 4 g .. + 3
-    for (int ob = 0; ob < 8; ++ob)
-      if (c == ob) {
-        uint32_t lo = (uint32_t)to_bf16(acc[ob][0] * (1.f / kAggC)) |
-                      ((uint32_t)to_bf16(acc[ob][1] * (1.f / kAggC)) << 16);
-        uint32_t hi = (uint32_t)to_bf16(acc[ob][2] * (1.f / kAggC)) |
-                      ((uint32_t)to_bf16(acc[ob][3] * (1.f / kAggC)) << 16);
-        *(uint2*)(S + (size_t)i * kWidth + 16 * ob + 4 * g) = make_uint2(lo, hi);
-      }
-    if (lane == 0) degc[i] = (float)deg / kAggC;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Fused node MLPs for inference. A wave handles tiles of 16 particles; each
-// layer runs transposed on 16x16x32 bf16 matrix cores (outputs as rows,
-// particles as columns), chaining in registers as the edge kernel does. B
-// operands come either from memory in the natural K order (slot t of k-step
-// ks, lane group g: feature 32 ks + 8 g + t) or from the previous layer's
-// registers in the permuted order (32 ks + 16 (t / 4) + 4 g + t % 4); each
-// layer's weights are staged once per work-group in LDS in the matching
-// order. Rounding points are those of the GEMM path (bf16 layer inputs,
-// float32 accumulation).
-constexpr int kNodeTile = 16;
-
-// Stages W rows for k-steps [ks0, ks0 + nks) of a layer with `ks_total`
-// k-steps and `nob` 16-wide output blocks: dst[(ob * ks_total + ks) * 64 +
-// lane] holds W[k][16 ob + lane % 16] for the lane's 8 K slots; rows at or
-// beyond `kvalid` are zero. `krow0` is the W row of the first staged k-step.
-__device__ __attribute__((always_inline)) inline void stage_layer(u32x4* dst, int ks_total, int ks0, int nks, int nob,
-                                   const uint16_t* __restrict__ Wm, int ld, int krow0, int kvalid,
-                                   bool perm) {
+// Stages W (row-major, rows = inputs, ld = outputs) for k-steps
+// [ks0, ks0 + nks) of a layer with `ks_total` k-steps and `nob` 16-wide
+// output blocks: dst[(ob * ks_total + ks) * 64 + lane] holds the lane's 8 K
+// slots of W^T row 16 ob + lane % 16, natural or permuted K order; input
+// rows at or beyond `kvalid` are zero. `krow0` is the W row of k-step ks0.
+__device__ __attribute__((always_inline)) inline void stage_layer(
+    u32x4* dst, int ks_total, int ks0, int nks, int nob, const uint16_t* __restrict__ Wm, int ld,
+    int krow0, int kvalid, bool perm) {
   const int total = nob * nks * 64;
   for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
     int l = idx & 63, ks = (idx >> 6) % nks, ob = (idx >> 6) / nks, g = l >> 4;
@@ -533,253 +402,179 @@ __device__ __attribute__((always_inline)) inline void stage_layer(u32x4* dst, in
 }
 
 template <int NOB, int NKS>
-__device__ __attribute__((always_inline)) inline void mfma_layer(const u32x4* __restrict__ w, const bf16x8* bop, f32x4* y) {
-  const int lane = threadIdx.x & 63;
+__device__ __attribute__((always_inline)) inline void mfma_layer(const u32x4* __restrict__ w,
+                                                                 const bf16x8* bop,
+                                                                 const float* __restrict__ bias,
+                                                                 f32x4* y) {
+  const int lane = threadIdx.x & 63, g = lane >> 4;
 #pragma unroll
   for (int ob = 0; ob < NOB; ++ob) {
-    f32x4 a = f32x4{0.f, 0.f, 0.f, 0.f};
+    f32x4 a;
+    for (int r = 0; r < 4; ++r) a[r] = bias[16 * ob + 4 * g + r];
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks)
-      a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, w[(ob * NKS + ks) * 64 + lane]),
-                                                   bop[ks], a, 0, 0, 0);
+      a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+          __builtin_bit_cast(bf16x8, w[(ob * NKS + ks) * 64 + lane]), bop[ks], a, 0, 0, 0);
     y[ob] = a;
-    // keep each block's weight reads next to its products: hoisting all of
-    // a layer's LDS reads would need 4 NOB NKS registers more
-    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-// Permuted-order B operands (4 k-steps, 128 features) from a 128-wide
-// layer's registers, optionally through SiLU.
-__device__ __attribute__((always_inline)) inline void to_operands(const f32x4* y, bool act, bf16x8* bop) {
+// Permuted-order B operands (NOB / 2 k-steps) from a layer's registers,
+// through SiLU.
+template <int NOB>
+__device__ __attribute__((always_inline)) inline void silu_operands(const f32x4* y, bf16x8* bop) {
 #pragma unroll
-  for (int ks = 0; ks < 4; ++ks) {
+  for (int ks = 0; ks < NOB / 2; ++ks) {
     float v8[8];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      float v = y[2 * ks + (t >> 2)][t & 3];
-      v8[t] = act ? silu(v) : v;
-    }
+    for (int t = 0; t < 8; ++t) v8[t] = silu(y[2 * ks + (t >> 2)][t & 3]);
     bop[ks] = __builtin_bit_cast(bf16x8, pack8(v8));
   }
 }
 
-// Natural-order B operands: features 32 ks + 8 g .. + 7 of a bf16 row.
-__device__ __attribute__((always_inline)) inline void load_operands(const uint16_t* __restrict__ row, bool valid, int nks,
-                                     bf16x8* bop) {
-  const int g = (threadIdx.x & 63) >> 4;
-  for (int ks = 0; ks < nks; ++ks) {
-    u32x4 v = {0u, 0u, 0u, 0u};
-    if (valid) v = *(const u32x4*)(row + 32 * ks + 8 * g);
-    bop[ks] = __builtin_bit_cast(bf16x8, v);
+// One round of 4 edge steps for the wave's tile (see above). MASKED rounds
+// contain edges past some column's degree: those edges store zero inputs and
+// their steps start from a zero accumulator, so they add relu(0) = 0.
+template <bool MASKED>
+__device__ __attribute__((always_inline)) inline void edge_round(
+    uint32_t k, uint32_t i, uint32_t n, uint32_t deg, const float4& ai, const float4& bi,
+    const float2& ni, const float4* __restrict__ fa, const float4* __restrict__ fb,
+    const float2* __restrict__ nu, const uint4* __restrict__ nbr, const EdgeConsts& kc,
+    uint8_t* wlds, const u32x4* __restrict__ w1s, const f32x4* zb, f32x4* s) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  // this lane's edge: 4 k + g of particle c
+  const bool mine = !MASKED || 4 * k + g < deg;
+  uint32_t j = i;
+  if (mine) {
+    uint4 js = nbr[(size_t)k * n + i];
+    j = g == 0 ? js.x : g == 1 ? js.y : g == 2 ? js.z : js.w;
   }
-}
-
-// Adds bias[16 ob + 4 g + r] (+ scale * bias2) to a 128-wide layer.
-__device__ __attribute__((always_inline)) inline void add_bias(f32x4* y, int nob, const float* __restrict__ b,
-                                const float* __restrict__ b2, float scale) {
-  const int g = (threadIdx.x & 63) >> 4;
-  for (int ob = 0; ob < nob; ++ob)
-    for (int r = 0; r < 4; ++r) {
-      int f = 16 * ob + 4 * g + r;
-      y[ob][r] += b[f] + (b2 ? scale * b2[f] : 0.f);
+  float f[kFeat];
+  edge_features(ai, bi, fa[j], fb[j], ni, nu[j], kc, f);
+  uint8_t* col = wlds + c * kColBytes;
+  for (int q = 0; q < 3; ++q) {
+    u32x4 v = pack8(f + 8 * q);
+    if (MASKED && !mine) v = u32x4{0u, 0u, 0u, 0u};
+    *(u32x4*)(col + g * 48 + 16 * q) = v;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int gr = g < 3 ? g : 2;
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    const bf16x8 eb = __builtin_bit_cast(bf16x8, *(const u32x4*)(col + st * 48 + gr * 16));
+    const bool live = !MASKED || 4 * k + st < deg;
+#pragma unroll
+    for (int ob = 0; ob < kEdgeW / 16; ++ob) {
+      f32x4 c0 = zb[ob];
+      if (MASKED && !live) c0 = f32x4{0.f, 0.f, 0.f, 0.f};
+      f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+          __builtin_bit_cast(bf16x8, w1s[ob * 64 + lane]), eb, c0, 0, 0, 0);
+      for (int r = 0; r < 4; ++r) s[ob][r] += fmaxf(z[r], 0.f);
     }
-}
-
-// Writes features 16 ob + 4 g .. + 3 of a particle's bf16 row.
-__device__ __attribute__((always_inline)) inline void store_row_bf16(uint16_t* __restrict__ row, const f32x4* y) {
-  const int g = (threadIdx.x & 63) >> 4;
-  for (int ob = 0; ob < 8; ++ob) {
-    uint32_t lo = (uint32_t)to_bf16(y[ob][0]) | ((uint32_t)to_bf16(y[ob][1]) << 16);
-    uint32_t hi = (uint32_t)to_bf16(y[ob][2]) | ((uint32_t)to_bf16(y[ob][3]) << 16);
-    *(uint2*)(row + 16 * ob + 4 * g) = make_uint2(lo, hi);
   }
+  // the next round's stores come after this round's reads
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Round-1 node part: M1 = S0 W3 + (deg / C) b3; H = MLP([M1, nu, 0]); Hb.
-__global__ __launch_bounds__(512) void k_node1(const uint16_t* __restrict__ S0,
-                                               const float* __restrict__ degc,
-                                               const float* __restrict__ nu, uint32_t n,
-                                               const uint16_t* __restrict__ wb,
-                                               const float* __restrict__ params,
-                                               uint16_t* __restrict__ Hb) {
-  extern __shared__ u32x4 lds[];
-  u32x4* w3 = lds;                 // 8 x 4
-  u32x4* wa = w3 + 8 * 4 * 64;     // 8 x 5
-  u32x4* wb2 = wa + 8 * 5 * 64;    // 8 x 4
-  u32x4* wc = wb2 + 8 * 4 * 64;    // 8 x 4
-  float* bias = (float*)(wc + 8 * 4 * 64);  // b3, ba, bb, bc
-  stage_layer(w3, 4, 0, 4, 8, wb + param_offset(E1_W3), kWidth, 0, kWidth, false);
-  stage_layer(wa, 5, 0, 4, 8, wb + param_offset(N1_W1), kWidth, 0, kIn1, true);
-  stage_layer(wa, 5, 4, 1, 8, wb + param_offset(N1_W1), kWidth, kWidth, kWidth + 2, false);
-  stage_layer(wb2, 4, 0, 4, 8, wb + param_offset(N1_W2), kWidth, 0, kWidth, true);
-  stage_layer(wc, 4, 0, 4, 8, wb + param_offset(N1_W3), kWidth, 0, kWidth, true);
-  for (int k = threadIdx.x; k < kWidth; k += blockDim.x) {
-    bias[k] = params[param_offset(E1_B3) + k];
-    bias[kWidth + k] = params[param_offset(N1_B1) + k];
-    bias[2 * kWidth + k] = params[param_offset(N1_B2) + k];
-    bias[3 * kWidth + k] = params[param_offset(N1_B3) + k];
+__global__ __launch_bounds__(kSurWaves * 64) void k_surrogate(
+    const float4* __restrict__ fa, const float4* __restrict__ fb, const float2* __restrict__ nu,
+    const uint4* __restrict__ nbr, const uint32_t* __restrict__ count, uint32_t n, EdgeConsts kc,
+    const uint16_t* __restrict__ wb, const float* __restrict__ params, float4* __restrict__ acc,
+    float* __restrict__ out, float gx, float gy, float gz) {
+  __shared__ u32x4 w1s[2 * 64];          // edge layer: 2 output blocks, 1 k-step
+  __shared__ u32x4 wn1[4 * 2 * 64];      // node layer 1: k-step 0 permuted, 1 natural
+  __shared__ u32x4 wn2[4 * 2 * 64];
+  __shared__ u32x4 wn3[2 * 2 * 64];
+  __shared__ float bias[kEdgeW + 2 * kNodeW + kOut];
+  __shared__ __attribute__((aligned(16))) uint8_t tiles_lds[kSurWaves * kWaveLds];
+  stage_layer(w1s, 1, 0, 1, 2, wb + param_offset(E_W1), kEdgeW, 0, kFeatUsed, false);
+  stage_layer(wn1, 2, 0, 1, 4, wb + param_offset(N_W1), kNodeW, 0, kEdgeW, true);
+  stage_layer(wn1, 2, 1, 1, 4, wb + param_offset(N_W1), kNodeW, kEdgeW, kEdgeW + 6, false);
+  stage_layer(wn2, 2, 0, 2, 4, wb + param_offset(N_W2), kNodeW, 0, kNodeW, true);
+  stage_layer(wn3, 2, 0, 2, 2, wb + param_offset(N_W3), kOut, 0, kNodeW, true);
+  for (int k = threadIdx.x; k < kEdgeW + 2 * kNodeW + kOut; k += blockDim.x) {
+    float v;
+    if (k < kEdgeW) v = params[param_offset(E_B1) + k];
+    else if (k < kEdgeW + kNodeW) v = params[param_offset(N_B1) + k - kEdgeW];
+    else if (k < kEdgeW + 2 * kNodeW) v = params[param_offset(N_B2) + k - kEdgeW - kNodeW];
+    else v = params[param_offset(N_B3) + k - kEdgeW - 2 * kNodeW];
+    bias[k] = v;
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
-  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-  const uint32_t tiles = (n + kNodeTile - 1) / kNodeTile;
-  for (uint32_t tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < tiles;
-       tile += waves) {
-    const uint32_t i = tile * kNodeTile + c;
-    const bool valid = i < n;
-    bf16x8 bop[5];
-    f32x4 y[8];
-    load_operands(S0 + (size_t)(valid ? i : 0) * kWidth, valid, 4, bop);
-    mfma_layer<8, 4>(w3, bop, y);
-    {
-      const float dc = valid ? degc[i] : 0.f;  // M1 = S0 W3 + (deg / C) b3
-      for (int ob = 0; ob < 8; ++ob)
-        for (int r = 0; r < 4; ++r) y[ob][r] += dc * bias[16 * ob + 4 * g + r];
+
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15, wave = threadIdx.x >> 6;
+  uint8_t* wlds = tiles_lds + wave * kWaveLds;
+  f32x4 zb[2];
+  for (int ob = 0; ob < 2; ++ob)
+    for (int r = 0; r < 4; ++r) zb[ob][r] = bias[16 * ob + 4 * g + r];
+  const uint32_t waves = gridDim.x * kSurWaves;
+  const uint32_t ntiles = (n + kTileP - 1) / kTileP;
+  for (uint32_t tile = blockIdx.x * kSurWaves + wave; tile < ntiles; tile += waves) {
+    const uint32_t ip = tile * kTileP + c;
+    const bool valid = ip < n;
+    const uint32_t i = valid ? ip : n - 1;
+    const uint32_t deg = valid ? min(count[i], kMaxEdges) : 0u;
+    uint32_t dmin = deg, dmax = deg;
+    for (int o = 1; o < 16; o <<= 1) {
+      dmin = min(dmin, (uint32_t)__shfl_xor((int)dmin, o, 64));
+      dmax = max(dmax, (uint32_t)__shfl_xor((int)dmax, o, 64));
     }
-    to_operands(y, false, bop);
+    const float4 ai = fa[i], bi = fb[i];
+    const float2 ni = nu[i];
+    f32x4 s[2];
+    for (int ob = 0; ob < 2; ++ob) s[ob] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const uint32_t kfull = dmin >> 2, kend = (dmax + 3) >> 2;
+    for (uint32_t k = 0; k < kfull; ++k)
+      edge_round<false>(k, i, n, deg, ai, bi, ni, fa, fb, nu, nbr, kc, wlds, w1s, zb, s);
+    for (uint32_t k = kfull; k < kend; ++k)
+      edge_round<true>(k, i, n, deg, ai, bi, ni, fa, fb, nu, nbr, kc, wlds, w1s, zb, s);
+
+    // node layer 1: [S, nu, (a0 - g) / A, deg / C, 0...]
+    const float4 a0 = acc ? acc[i] : make_float4(gx, gy, gz, 0.f);
+    bf16x8 bop[2];
     {
-      float v8[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      if (g == 0 && valid) {
-        v8[0] = nu[2 * i];
-        v8[1] = nu[2 * i + 1];
+      float v8[8];
+      for (int t = 0; t < 8; ++t) v8[t] = s[t >> 2][t & 3] * (1.f / kAggC);
+      bop[0] = __builtin_bit_cast(bf16x8, pack8(v8));
+      for (int t = 0; t < 8; ++t) v8[t] = 0.f;
+      if (g == 0) {
+        constexpr float inv_a = 1.f / kAccScale;
+        v8[0] = ni.x, v8[1] = ni.y;
+        v8[2] = (a0.x - gx) * inv_a, v8[3] = (a0.y - gy) * inv_a, v8[4] = (a0.z - gz) * inv_a;
+        v8[5] = (float)deg * (1.f / kAggC);
       }
-      bop[4] = __builtin_bit_cast(bf16x8, pack8(v8));
+      bop[1] = __builtin_bit_cast(bf16x8, pack8(v8));
     }
-    mfma_layer<8, 5>(wa, bop, y);
-    add_bias(y, 8, bias + kWidth, nullptr, 0.f);
-    to_operands(y, true, bop);
-    mfma_layer<8, 4>(wb2, bop, y);
-    add_bias(y, 8, bias + 2 * kWidth, nullptr, 0.f);
-    to_operands(y, true, bop);
-    mfma_layer<8, 4>(wc, bop, y);
-    add_bias(y, 8, bias + 3 * kWidth, nullptr, 0.f);
-    if (valid) store_row_bf16(Hb + (size_t)i * kWidth, y);
-  }
-}
-
-// Round-2 node terms: PA = bf16(Hb Wa), PB = bf16(Hb Wb).
-__global__ __launch_bounds__(512) void k_node_terms(const uint16_t* __restrict__ Hb, uint32_t n,
-                                                    const uint16_t* __restrict__ wb,
-                                                    uint16_t* __restrict__ PA,
-                                                    uint16_t* __restrict__ PB) {
-  extern __shared__ u32x4 lds[];
-  u32x4* wa = lds;
-  u32x4* wbb = wa + 8 * 4 * 64;
-  stage_layer(wa, 4, 0, 4, 8, wb + param_offset(E2_WA), kWidth, 0, kWidth, false);
-  stage_layer(wbb, 4, 0, 4, 8, wb + param_offset(E2_WB), kWidth, 0, kWidth, false);
-  __syncthreads();
-  const int c = threadIdx.x & 15;
-  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-  const uint32_t tiles = (n + kNodeTile - 1) / kNodeTile;
-  for (uint32_t tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < tiles;
-       tile += waves) {
-    const uint32_t i = tile * kNodeTile + c;
-    const bool valid = i < n;
-    bf16x8 bop[4];
-    f32x4 y[8];
-    load_operands(Hb + (size_t)(valid ? i : 0) * kWidth, valid, 4, bop);
-    mfma_layer<8, 4>(wa, bop, y);
-    if (valid) store_row_bf16(PA + (size_t)i * kWidth, y);
-    mfma_layer<8, 4>(wbb, bop, y);
-    if (valid) store_row_bf16(PB + (size_t)i * kWidth, y);
-  }
-}
-
-// Round-2 node part: M2 = S1 W3' + (deg / C) b3'; out = MLP([H, M2]);
-// acc = g + A out[0:3]. Writes out (32 floats per particle) as well.
-__global__ __launch_bounds__(512) void k_node2(const uint16_t* __restrict__ S1,
-                                               const uint16_t* __restrict__ Hb,
-                                               const float* __restrict__ degc, uint32_t n,
-                                               const uint16_t* __restrict__ wb,
-                                               const float* __restrict__ params,
-                                               float* __restrict__ out, float4* __restrict__ acc,
-                                               float gx, float gy, float gz) {
-  extern __shared__ u32x4 lds[];
-  u32x4* w3 = lds;                 // 8 x 4
-  u32x4* wa = w3 + 8 * 4 * 64;     // 8 x 8
-  u32x4* wb2 = wa + 8 * 8 * 64;    // 8 x 4
-  u32x4* wc = wb2 + 8 * 4 * 64;    // 2 x 4
-  float* bias = (float*)(wc + 2 * 4 * 64);  // b3, ba, bb (128 each), bc (32)
-  stage_layer(w3, 4, 0, 4, 8, wb + param_offset(E2_W3), kWidth, 0, kWidth, false);
-  stage_layer(wa, 8, 0, 4, 8, wb + param_offset(N2_W1), kWidth, 0, kWidth, false);
-  stage_layer(wa, 8, 4, 4, 8, wb + param_offset(N2_W1), kWidth, kWidth, kIn2, true);
-  stage_layer(wb2, 4, 0, 4, 8, wb + param_offset(N2_W2), kWidth, 0, kWidth, true);
-  stage_layer(wc, 4, 0, 4, 2, wb + param_offset(N2_W3), kOut, 0, kWidth, true);
-  for (int k = threadIdx.x; k < kWidth; k += blockDim.x) {
-    bias[k] = params[param_offset(E2_B3) + k];
-    bias[kWidth + k] = params[param_offset(N2_B1) + k];
-    bias[2 * kWidth + k] = params[param_offset(N2_B2) + k];
-    if (k < kOut) bias[3 * kWidth + k] = params[param_offset(N2_B3) + k];
-  }
-  __syncthreads();
-  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
-  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-  const uint32_t tiles = (n + kNodeTile - 1) / kNodeTile;
-  for (uint32_t tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < tiles;
-       tile += waves) {
-    const uint32_t i = tile * kNodeTile + c;
-    const bool valid = i < n;
-    bf16x8 bop[8];
-    f32x4 y[8];
-    load_operands(S1 + (size_t)(valid ? i : 0) * kWidth, valid, 4, bop);
-    mfma_layer<8, 4>(w3, bop, y);
-    {
-      const float dc = valid ? degc[i] : 0.f;
-      for (int ob = 0; ob < 8; ++ob)
-        for (int r = 0; r < 4; ++r) y[ob][r] += dc * bias[16 * ob + 4 * g + r];
-    }
-    load_operands(Hb + (size_t)(valid ? i : 0) * kWidth, valid, 4, bop);
-    to_operands(y, false, bop + 4);
-    mfma_layer<8, 8>(wa, bop, y);
-    add_bias(y, 8, bias + kWidth, nullptr, 0.f);
-    to_operands(y, true, bop);
-    mfma_layer<8, 4>(wb2, bop, y);
-    add_bias(y, 8, bias + 2 * kWidth, nullptr, 0.f);
-    to_operands(y, true, bop);
+    f32x4 y[4];
+    mfma_layer<4, 2>(wn1, bop, bias + kEdgeW, y);
+    silu_operands<4>(y, bop);
+    mfma_layer<4, 2>(wn2, bop, bias + kEdgeW + kNodeW, y);
+    silu_operands<4>(y, bop);
     f32x4 o[2];
-    mfma_layer<2, 4>(wc, bop, o);
-    for (int ob = 0; ob < 2; ++ob)
-      for (int r = 0; r < 4; ++r) o[ob][r] += bias[3 * kWidth + 16 * ob + 4 * g + r];
+    mfma_layer<2, 2>(wn3, bop, bias + kEdgeW + 2 * kNodeW, o);
     if (valid) {
-      float* orow = out + (size_t)i * kOut;
-      for (int ob = 0; ob < 2; ++ob)
-        *(float4*)(orow + 16 * ob + 4 * g) = make_float4(o[ob][0], o[ob][1], o[ob][2], o[ob][3]);
-      if (g == 0)
-        acc[i] = make_float4(gx + kAccScale * o[0][0], gy + kAccScale * o[0][1],
-                             gz + kAccScale * o[0][2], 0.f);
+      if (out) {
+        float* orow = out + (size_t)i * kOut;
+        for (int ob = 0; ob < 2; ++ob)
+          *(float4*)(orow + 16 * ob + 4 * g) = make_float4(o[ob][0], o[ob][1], o[ob][2], o[ob][3]);
+      }
+      if (acc && g == 0)
+        acc[i] = make_float4(a0.x + kResScale * o[0][0], a0.y + kResScale * o[0][1],
+                             a0.z + kResScale * o[0][2], a0.w);
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Elementwise and row kernels (grid-stride loops).
+// Training-path kernels (grid-stride loops unless noted).
 __global__ void k_silu_bf16(const float* __restrict__ z, uint16_t* __restrict__ s, size_t count) {
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < count;
        k += (size_t)gridDim.x * blockDim.x)
     s[k] = to_bf16(silu(z[k]));
-}
-
-__global__ void k_cast_2d(const float* __restrict__ x, int ldx, uint16_t* __restrict__ y, int ldy,
-                          size_t rows, int cols) {
-  size_t count = rows * cols;
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < count;
-       k += (size_t)gridDim.x * blockDim.x) {
-    size_t r = k / cols;
-    int c = (int)(k % cols);
-    y[r * ldy + c] = to_bf16(x[r * ldx + c]);
-  }
-}
-
-__global__ void k_copy_2d(const float* __restrict__ x, int ldx, float* __restrict__ y, int ldy,
-                          size_t rows, int cols) {
-  size_t count = rows * cols;
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < count;
-       k += (size_t)gridDim.x * blockDim.x) {
-    size_t r = k / cols;
-    int c = (int)(k % cols);
-    y[r * ldy + c] = x[r * ldx + c];
-  }
 }
 
 // dz = bf16(dy * silu'(p)); dy has row stride ldy, p and dz are dense.
@@ -794,79 +589,59 @@ __global__ void k_dsilu(const float* __restrict__ dy, int ldy, const float* __re
   }
 }
 
-// dz2[e] = bf16(dS[dst[e]] * silu'(z2[e])), 128 features per edge.
-__global__ void k_edge_dz(const float* __restrict__ dS, const float* __restrict__ z2,
-                          const uint32_t* __restrict__ dst, uint16_t* __restrict__ dz,
-                          size_t e) {
-  size_t count = e * kWidth;
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < count;
-       k += (size_t)gridDim.x * blockDim.x) {
-    size_t ed = k / kWidth;
-    int c = (int)(k % kWidth);
-    dz[k] = to_bf16(dS[(size_t)dst[ed] * kWidth + c] * dsilu(z2[k]));
-  }
-}
-
-// z1[e] += PA[dst[e]] + PB[src[e]] (bf16 node terms of round 2).
-__global__ void k_edge_add(float* __restrict__ z1, const uint16_t* __restrict__ PA,
-                           const uint16_t* __restrict__ PB, const uint32_t* __restrict__ src,
-                           const uint32_t* __restrict__ dst, size_t e) {
-  size_t count = e * kWidth;
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < count;
-       k += (size_t)gridDim.x * blockDim.x) {
-    size_t ed = k / kWidth;
-    int c = (int)(k % kWidth);
-    float z = z1[k] + from_bf16(PA[(size_t)dst[ed] * kWidth + c]);
-    z1[k] = z + from_bf16(PB[(size_t)src[ed] * kWidth + c]);
-  }
-}
-
-// S[i] = bf16(scale * sum over i's edges of silu(z2)); one wave per node.
-__global__ __launch_bounds__(256) void k_segsum_silu(const float* __restrict__ z2,
+// S[i][c] = bf16(sum over i's edges of relu(z[e][c]) / C); thread per (i, c).
+__global__ __launch_bounds__(256) void k_segsum_relu(const float* __restrict__ z,
                                                      const uint32_t* __restrict__ off, uint32_t n,
-                                                     float scale, uint16_t* __restrict__ S) {
-  uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
-  int lane = threadIdx.x & 63;
-  if (i >= n) return;
-  float a0 = 0.f, a1 = 0.f;
-  for (uint32_t k = off[i]; k < off[i + 1]; ++k) {
-    float2 z = *(const float2*)(z2 + (size_t)k * kWidth + 2 * lane);
-    a0 += silu(z.x);
-    a1 += silu(z.y);
+                                                     uint16_t* __restrict__ S) {
+  const size_t count = (size_t)n * kEdgeW;
+  for (size_t t = blockIdx.x * (size_t)256 + threadIdx.x; t < count;
+       t += (size_t)gridDim.x * 256) {
+    uint32_t i = (uint32_t)(t / kEdgeW);
+    int c = (int)(t % kEdgeW);
+    float a = 0.f;
+    for (uint32_t k = off[i]; k < off[i + 1]; ++k) a += fmaxf(z[(size_t)k * kEdgeW + c], 0.f);
+    S[t] = to_bf16(a * (1.f / kAggC));
   }
-  S[(size_t)i * kWidth + 2 * lane] = to_bf16(a0 * scale);
-  S[(size_t)i * kWidth + 2 * lane + 1] = to_bf16(a1 * scale);
 }
 
-// out[i] = sum over i's edges of x (bf16 rows, 128 wide); one wave per node.
-__global__ __launch_bounds__(256) void k_segsum_rows(const uint16_t* __restrict__ x,
-                                                     const uint32_t* __restrict__ off, uint32_t n,
-                                                     float* __restrict__ out) {
-  uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
-  int lane = threadIdx.x & 63;
-  if (i >= n) return;
-  float a0 = 0.f, a1 = 0.f;
-  for (uint32_t k = off[i]; k < off[i + 1]; ++k) {
-    uint32_t v = *(const uint32_t*)(x + (size_t)k * kWidth + 2 * lane);
-    a0 += from_bf16((uint16_t)(v & 0xFFFFu));
-    a1 += from_bf16((uint16_t)(v >> 16));
-  }
-  out[(size_t)i * kWidth + 2 * lane] = a0;
-  out[(size_t)i * kWidth + 2 * lane + 1] = a1;
-}
-
-// out[src[e]] += x[e] (atomics; 128 wide).
-__global__ void k_scatter_rows(const uint16_t* __restrict__ x, const uint32_t* __restrict__ src,
-                               size_t e, float* __restrict__ out) {
-  size_t count = e * (kWidth / 2);
+// u = bf16([S, nu, (a0 - g) / A, deg / C, 0...]) (64 wide).
+__global__ void k_pack_u(const uint16_t* __restrict__ S, const float* __restrict__ nu,
+                         const float4* __restrict__ acc0, const float* __restrict__ degc,
+                         uint32_t n, float gx, float gy, float gz, uint16_t* __restrict__ u) {
+  size_t count = (size_t)n * kIn;
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < count;
        k += (size_t)gridDim.x * blockDim.x) {
-    size_t ed = k / (kWidth / 2);
-    int c = 2 * (int)(k % (kWidth / 2));
-    uint32_t v = *(const uint32_t*)(x + ed * kWidth + c);
-    float* o = out + (size_t)src[ed] * kWidth + c;
-    atomicAdd(o, from_bf16((uint16_t)(v & 0xFFFFu)));
-    atomicAdd(o + 1, from_bf16((uint16_t)(v >> 16)));
+    size_t i = k / kIn;
+    int c = (int)(k % kIn);
+    if (c < kEdgeW) {
+      u[k] = S[i * kEdgeW + c];
+      continue;
+    }
+    float v = 0.f;
+    const float4 a = acc0[i];
+    switch (c - kEdgeW) {
+      case 0: v = nu[2 * i]; break;
+      case 1: v = nu[2 * i + 1]; break;
+      case 2: v = (a.x - gx) * (1.f / kAccScale); break;
+      case 3: v = (a.y - gy) * (1.f / kAccScale); break;
+      case 4: v = (a.z - gz) * (1.f / kAccScale); break;
+      case 5: v = degc[i]; break;
+      default: break;
+    }
+    u[k] = to_bf16(v);
+  }
+}
+
+// dz[e][c] = bf16(dY[dst[e]][c] / C) where z[e][c] > 0, else 0.
+__global__ void k_edge_dz(const float* __restrict__ dY, int ldy, const float* __restrict__ z,
+                          const uint32_t* __restrict__ dst, uint16_t* __restrict__ dz, size_t e) {
+  size_t count = e * kEdgeW;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < count;
+       k += (size_t)gridDim.x * blockDim.x) {
+    size_t ed = k / kEdgeW;
+    int c = (int)(k % kEdgeW);
+    float d = z[k] > 0.f ? dY[(size_t)dst[ed] * ldy + c] * (1.f / kAggC) : 0.f;
+    dz[k] = to_bf16(d);
   }
 }
 
@@ -881,41 +656,13 @@ __device__ inline float load_val<uint16_t>(const uint16_t* p) { return from_bf16
 
 template <typename T>
 __global__ void k_colsum(const T* __restrict__ x, int ldx, size_t rows, int cols,
-                         const float* __restrict__ rowscale, float* __restrict__ out) {
+                         float* __restrict__ out) {
   int c = threadIdx.x;
   if (c >= cols) return;
   size_t r0 = (size_t)blockIdx.x * 512, r1 = r0 + 512 < rows ? r0 + 512 : rows;
   float a = 0.f;
-  for (size_t r = r0; r < r1; ++r) {
-    float v = load_val<T>(x + r * ldx + c);
-    a += rowscale ? v * rowscale[r] : v;
-  }
+  for (size_t r = r0; r < r1; ++r) a += load_val<T>(x + r * ldx + c);
   atomicAdd(out + c, a);
-}
-
-// u1 = bf16([M1, nu, 0...]) (144 wide).
-__global__ void k_pack_u1(const float* __restrict__ M, const float* __restrict__ nu, uint32_t n,
-                          uint16_t* __restrict__ u1) {
-  size_t count = (size_t)n * kIn1;
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < count;
-       k += (size_t)gridDim.x * blockDim.x) {
-    size_t i = k / kIn1;
-    int c = (int)(k % kIn1);
-    float v = c < kWidth ? M[i * kWidth + c] : (c < kWidth + 2 ? nu[2 * i + c - kWidth] : 0.f);
-    u1[k] = to_bf16(v);
-  }
-}
-
-// u2 = [H (bf16), bf16(M2)] (256 wide).
-__global__ void k_pack_u2(const uint16_t* __restrict__ Hb, const float* __restrict__ M2,
-                          uint32_t n, uint16_t* __restrict__ u2) {
-  size_t count = (size_t)n * kIn2;
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < count;
-       k += (size_t)gridDim.x * blockDim.x) {
-    size_t i = k / kIn2;
-    int c = (int)(k % kIn2);
-    u2[k] = c < kWidth ? Hb[i * kWidth + c] : to_bf16(M2[i * kWidth + c - kWidth]);
-  }
 }
 
 // Loss = sum over particles and the 3 used outputs of (out - target)^2 / n;
@@ -968,15 +715,6 @@ __global__ void k_cast(const float* __restrict__ x, uint16_t* __restrict__ y, si
     y[k] = to_bf16(x[k]);
 }
 
-// acc[i] = g + A * out[i, 0:3]
-__global__ void k_write_acc(const float* __restrict__ out, uint32_t n, float gx, float gy,
-                            float gz, float4* __restrict__ acc) {
-  uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const float* o = out + (size_t)i * kOut;
-  acc[i] = make_float4(gx + kAccScale * o[0], gy + kAccScale * o[1], gz + kAccScale * o[2], 0.f);
-}
-
 // loc[ids[k]] = k
 __global__ void k_invert(const uint32_t* __restrict__ ids, uint32_t n, uint32_t* __restrict__ loc) {
   uint32_t k = blockIdx.x * 256 + threadIdx.x;
@@ -986,17 +724,18 @@ __global__ void k_invert(const uint32_t* __restrict__ ids, uint32_t n, uint32_t*
   }
 }
 
+// target[i] = ((vh_T - vh_S) / dt - a0_i) / R for the learner's sorted i.
 __global__ void k_target(const uint32_t* __restrict__ perm, const float4* __restrict__ vh_s,
                          const uint32_t* __restrict__ loc, const float4* __restrict__ vh_t,
-                         uint32_t n, float inv_dt, float gx, float gy, float gz,
+                         const float4* __restrict__ acc0, uint32_t n, float inv_dt,
                          float* __restrict__ target) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   uint32_t c = perm[i];
-  float4 a = vh_s[c], b = vh_t[loc[c]];
-  target[3 * i] = ((b.x - a.x) * inv_dt - gx) / kAccScale;
-  target[3 * i + 1] = ((b.y - a.y) * inv_dt - gy) / kAccScale;
-  target[3 * i + 2] = ((b.z - a.z) * inv_dt - gz) / kAccScale;
+  float4 a = vh_s[c], b = vh_t[loc[c]], a0 = acc0[i];
+  target[3 * i] = ((b.x - a.x) * inv_dt - a0.x) / kResScale;
+  target[3 * i + 1] = ((b.y - a.y) * inv_dt - a0.y) / kResScale;
+  target[3 * i + 2] = ((b.z - a.z) * inv_dt - a0.z) / kResScale;
 }
 
 // inv[perm[k]] = k
@@ -1056,17 +795,18 @@ __global__ void k_accum_acc(const float4* __restrict__ acc, const uint32_t* __re
   sum[3 * id + 2] += a.z;
 }
 
-// target[i] = (sum[perm[i]] / nsub - g) / A for the learner's sorted i.
+// target[i] = (sum[perm[i]] / nsub - a0_i) / R for the learner's sorted i.
 __global__ void k_target_mean(const uint32_t* __restrict__ perm, const float* __restrict__ sum,
-                              uint32_t n, float inv_nsub, float gx, float gy, float gz,
+                              const float4* __restrict__ acc0, uint32_t n, float inv_nsub,
                               float* __restrict__ target) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   uint32_t c = perm[i];
   if (c >= n) c = 0;
-  target[3 * i] = (sum[3 * c] * inv_nsub - gx) / kAccScale;
-  target[3 * i + 1] = (sum[3 * c + 1] * inv_nsub - gy) / kAccScale;
-  target[3 * i + 2] = (sum[3 * c + 2] * inv_nsub - gz) / kAccScale;
+  float4 a0 = acc0[i];
+  target[3 * i] = (sum[3 * c] * inv_nsub - a0.x) / kResScale;
+  target[3 * i + 1] = (sum[3 * c + 1] * inv_nsub - a0.y) / kResScale;
+  target[3 * i + 2] = (sum[3 * c + 2] * inv_nsub - a0.z) / kResScale;
 }
 
 // ---------------------------------------------------------------------------
@@ -1110,240 +850,98 @@ static inline const float* P(const clsph_nn* m, int id) { return m->params + par
 static inline const uint16_t* W(const clsph_nn* m, int id) { return m->wb + param_offset(id); }
 static inline float* G(clsph_nn* m, int id) { return m->grad + param_offset(id); }
 
-// Edge MLP of one round: z1 (already holding e W1 + b1 [+ node terms]) ->
-// s1 -> z2 -> S = bf16(sum silu(z2) / C).
-static void edge_tail(clsph_nn* m, int r, int w2, int b2, hipStream_t s) {
-  size_t ew = (size_t)m->e * kWidth;
-  hipLaunchKernelGGL(k_silu_bf16, grid_for(ew), dim3(256), 0, s, m->z1[r], m->s1[r], ew);
-  gemm(s, false, false, false, m->s1[r], kWidth, W(m, w2), kWidth, m->z2[r], kWidth, (int)m->e,
-       kWidth, kWidth, 1.f, P(m, b2));
-  hipLaunchKernelGGL(k_segsum_silu, dim3((m->n + 3) / 4), dim3(256), 0, s, m->z2[r], m->off,
-                     m->n, 1.f / kAggC, m->S[r]);
-}
-
-// Three-layer SiLU MLP over n rows: input u (bf16, kin wide) ->
-// pre1/act1 -> pre2/act2 -> y (f32, kout wide).
-static void mlp3(clsph_nn* m, const uint16_t* u, int kin, int w1, float* pre1, uint16_t* act1,
-                 float* pre2, uint16_t* act2, float* y, int kout, hipStream_t s) {
-  int n = (int)m->n;
-  size_t nw = (size_t)n * kWidth;
-  gemm(s, false, false, false, u, kin, W(m, w1), kWidth, pre1, kWidth, n, kWidth, kin, 1.f,
-       P(m, w1 + 1));
-  hipLaunchKernelGGL(k_silu_bf16, grid_for(nw), dim3(256), 0, s, pre1, act1, nw);
-  gemm(s, false, false, false, act1, kWidth, W(m, w1 + 2), kWidth, pre2, kWidth, n, kWidth,
-       kWidth, 1.f, P(m, w1 + 3));
-  hipLaunchKernelGGL(k_silu_bf16, grid_for(nw), dim3(256), 0, s, pre2, act2, nw);
-  gemm(s, false, false, false, act2, kWidth, W(m, w1 + 4), kout, y, kout, n, kout, kWidth, 1.f,
-       P(m, w1 + 5));
-}
-
-// Node part of round 1: M1 from S[0], node MLP 1 -> H (bf16), and the
-// round-2 node terms PA = bf16(H Wa), PB = bf16(H Wb).
-static void node_round1(clsph_nn* m, hipStream_t s) {
-  int n = (int)m->n;
-  size_t nw = (size_t)n * kWidth;
-  gemm(s, false, false, false, m->S[0], kWidth, W(m, E1_W3), kWidth, m->tmp, kWidth, n, kWidth,
-       kWidth, 1.f, P(m, E1_B3), m->degc);
-  hipLaunchKernelGGL(k_pack_u1, grid_for((size_t)n * kIn1), dim3(256), 0, s, m->tmp, m->nu,
-                     m->n, m->u1);
-  mlp3(m, m->u1, kIn1, N1_W1, m->p1, m->y1, m->p2, m->y2, m->tmp, kWidth, s);
-  hipLaunchKernelGGL(k_cast, grid_for(nw), dim3(256), 0, s, m->tmp, m->Hb, nw);
-  gemm(s, false, false, false, m->Hb, kWidth, W(m, E2_WA), kWidth, m->tmp, kWidth, n, kWidth,
-       kWidth, 1.f);
-  hipLaunchKernelGGL(k_cast, grid_for(nw), dim3(256), 0, s, m->tmp, m->PA, nw);
-  gemm(s, false, false, false, m->Hb, kWidth, W(m, E2_WB), kWidth, m->tmp, kWidth, n, kWidth,
-       kWidth, 1.f);
-  hipLaunchKernelGGL(k_cast, grid_for(nw), dim3(256), 0, s, m->tmp, m->PB, nw);
-}
-
-// Node part of round 2: M2 from S[1], node MLP 2 -> out, accelerations.
-static void node_round2(clsph_nn* m, float4* acc, float gx, float gy, float gz, hipStream_t s) {
-  int n = (int)m->n;
-  gemm(s, false, false, false, m->S[1], kWidth, W(m, E2_W3), kWidth, m->tmp, kWidth, n, kWidth,
-       kWidth, 1.f, P(m, E2_B3), m->degc);
-  hipLaunchKernelGGL(k_pack_u2, grid_for((size_t)n * kIn2), dim3(256), 0, s, m->Hb, m->tmp,
-                     m->n, m->u2);
-  mlp3(m, m->u2, kIn2, N2_W1, m->q1, m->w1, m->q2, m->w2, m->out, kOut, s);
-  if (acc)
-    hipLaunchKernelGGL(k_write_acc, grid_for(n), dim3(256), 0, s, m->out, m->n, gx, gy, gz, acc);
-}
-
-void forward(clsph_nn* m, const DevParams* dp, float4* acc, float gx, float gy, float gz) {
-  (void)dp;
+void forward(clsph_nn* m, const float4* acc0, float gx, float gy, float gz) {
   hipStream_t s = m->ctx->stream;
-  int e = (int)m->e;
-  // round 1 edges
-  gemm(s, false, false, false, m->E1, kFeat, W(m, E1_W1), kWidth, m->z1[0], kWidth, e, kWidth,
-       kFeat, 1.f, P(m, E1_B1));
-  edge_tail(m, 0, E1_W2, E1_B2, s);
-  node_round1(m, s);
-  // round 2 edges
-  gemm(s, false, false, false, m->E1, kFeat, W(m, E2_W1), kWidth, m->z1[1], kWidth, e, kWidth,
-       kFeat, 1.f, P(m, E2_B1));
-  size_t ew = (size_t)e * kWidth;
-  hipLaunchKernelGGL(k_edge_add, grid_for(ew), dim3(256), 0, s, m->z1[1], m->PA, m->PB, m->src,
-                     m->dst, (size_t)e);
-  edge_tail(m, 1, E2_W2, E2_B2, s);
-  node_round2(m, acc, gx, gy, gz, s);
+  int n = (int)m->n, e = (int)m->e;
+  size_t nw = (size_t)n * kNodeW;
+  gemm(s, false, false, false, m->E1, kFeat, W(m, E_W1), kEdgeW, m->z, kEdgeW, e, kEdgeW, kFeat,
+       1.f, P(m, E_B1));
+  hipLaunchKernelGGL(k_segsum_relu, grid_for((size_t)n * kEdgeW), dim3(256), 0, s, m->z, m->off,
+                     m->n, m->S);
+  hipLaunchKernelGGL(k_pack_u, grid_for((size_t)n * kIn), dim3(256), 0, s, m->S, m->nu, acc0,
+                     m->degc, m->n, gx, gy, gz, m->u);
+  gemm(s, false, false, false, m->u, kIn, W(m, N_W1), kNodeW, m->p1, kNodeW, n, kNodeW, kIn, 1.f,
+       P(m, N_B1));
+  hipLaunchKernelGGL(k_silu_bf16, grid_for(nw), dim3(256), 0, s, m->p1, m->y1, nw);
+  gemm(s, false, false, false, m->y1, kNodeW, W(m, N_W2), kNodeW, m->p2, kNodeW, n, kNodeW,
+       kNodeW, 1.f, P(m, N_B2));
+  hipLaunchKernelGGL(k_silu_bf16, grid_for(nw), dim3(256), 0, s, m->p2, m->y2, nw);
+  gemm(s, false, false, false, m->y2, kNodeW, W(m, N_W3), kOut, m->out, kOut, n, kOut, kNodeW,
+       1.f, P(m, N_B3));
 }
 
-void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float gx, float gy, float gz,
-                   hipEvent_t* ev) {
+void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float* out, float gx, float gy,
+                   float gz, hipEvent_t* ev) {
   hipStream_t s = m->ctx->stream;
   uint32_t n = st->n;
   m->n = n;
   const DevParams& d = st->host_dps[0];
-  // persistent grid: the work-groups that fit at once (three per compute
-  // unit at the kernels' register and LDS use) stage the weights once each
-  uint32_t blocks = (uint32_t)st->ctx->cus * 3;
-  if (blocks * 4 > n) blocks = (n + 3) / 4;
-  if (blocks == 0) blocks = 1;
   if (ev) (void)hipEventRecord(ev[0], s);
   hipLaunchKernelGGL(k_node_inputs, grid_for(n), dim3(256), 0, s, st->rho_s, st->prs_s, n,
                      d.p.fluid_density, d.p.K, (float2*)m->nu);
-  hipLaunchKernelGGL((k_edge_fused<false>), dim3(blocks), dim3(256), 0, s, st->fa, st->fb,
-                     (const float2*)m->nu, st->nbr, st->nbr_count, n, edge_consts(d),
-                     W(m, E1_W1), P(m, E1_B1), W(m, E1_W2), P(m, E1_B2),
-                     (const uint16_t*)nullptr, (const uint16_t*)nullptr, m->S[0], m->degc);
-  if (ev) (void)hipEventRecord(ev[1], s);
-  // node part of round 1 (fused)
-  constexpr size_t kLdsNode1 = (size_t)(8 * 4 + 8 * 5 + 8 * 4 + 8 * 4) * 64 * 16 + 4 * kWidth * 4;
-  constexpr size_t kLdsTerms = (size_t)(8 * 4 + 8 * 4) * 64 * 16;
-  constexpr size_t kLdsNode2 =
-      (size_t)(8 * 4 + 8 * 8 + 8 * 4 + 2 * 4) * 64 * 16 + (3 * kWidth + kOut) * 4;
-  static bool lds_set = false;
-  if (!lds_set) {
-    (void)hipFuncSetAttribute((const void*)k_node1, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kLdsNode1);
-    (void)hipFuncSetAttribute((const void*)k_node_terms,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsTerms);
-    (void)hipFuncSetAttribute((const void*)k_node2, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kLdsNode2);
-    lds_set = true;
+  // persistent grid: the work-groups that fit on the device at once (the
+  // kernel's registers and LDS allow two per compute unit) stage the weights
+  // once each
+  static int per_cu = 0;
+  if (per_cu <= 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_surrogate, kSurWaves * 64, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+      per_cu = 1;
   }
-  uint32_t tiles = (n + kNodeTile - 1) / kNodeTile;
-  uint32_t nblocks = (uint32_t)st->ctx->cus;
-  if (nblocks * 8 > tiles) nblocks = (tiles + 7) / 8;
-  if (nblocks == 0) nblocks = 1;
-  hipLaunchKernelGGL(k_node1, dim3(nblocks), dim3(512), kLdsNode1, s, m->S[0], m->degc, m->nu, n,
-                     m->wb, m->params, m->Hb);
-  hipLaunchKernelGGL(k_node_terms, dim3(nblocks), dim3(512), kLdsTerms, s, m->Hb, n, m->wb, m->PA,
-                     m->PB);
-  if (ev) (void)hipEventRecord(ev[2], s);
-  hipLaunchKernelGGL((k_edge_fused<true>), dim3(blocks), dim3(256), 0, s, st->fa, st->fb,
-                     (const float2*)m->nu, st->nbr, st->nbr_count, n, edge_consts(d),
-                     W(m, E2_W1), P(m, E2_B1), W(m, E2_W2), P(m, E2_B2), m->PA, m->PB,
-                     m->S[1], m->degc);
-  if (ev) (void)hipEventRecord(ev[3], s);
-  hipLaunchKernelGGL(k_node2, dim3(nblocks), dim3(512), kLdsNode2, s, m->S[1], m->Hb, m->degc, n,
-                     m->wb, m->params, m->out, acc, gx, gy, gz);
-  if (ev) (void)hipEventRecord(ev[4], s);
+  uint32_t ntiles = (n + kTileP - 1) / kTileP;
+  uint32_t blocks = (uint32_t)st->ctx->cus * (uint32_t)per_cu;
+  if (blocks * kSurWaves > ntiles) blocks = (ntiles + kSurWaves - 1) / kSurWaves;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_surrogate, dim3(blocks), dim3(kSurWaves * 64), 0, s, st->fa, st->fb,
+                     (const float2*)m->nu, st->nbr, st->nbr_count, n, edge_consts(d), m->wb,
+                     m->params, acc, out, gx, gy, gz);
+  if (ev) (void)hipEventRecord(ev[1], s);
 }
 
-// Backward of a three-layer MLP given dy (f32, kout wide, in dY-compatible
-// storage `dy` with stride ldy). Leaves d(input) in m->dY (f32, kin_used wide).
-static void mlp3_back(clsph_nn* m, int w1, const uint16_t* u, int kin, const float* pre1,
-                      const uint16_t* act1, const float* pre2, const uint16_t* act2,
-                      const uint16_t* dy_b, const float* dy, int ldy, int kout, int kin_used,
-                      hipStream_t s) {
-  int n = (int)m->n;
-  // layer 3
-  gemm(s, true, false, true, act2, kWidth, dy_b, kout, G(m, w1 + 4), kout, kWidth, kout, n, 1.f);
-  hipLaunchKernelGGL((k_colsum<float>), dim3((n + 511) / 512), dim3(kout), 0, s, dy, ldy,
-                     (size_t)n, kout, nullptr, G(m, w1 + 5));
-  gemm(s, false, true, false, dy_b, kout, W(m, w1 + 4), kout, m->dY, kWidth, n, kWidth, kout, 1.f);
-  hipLaunchKernelGGL(k_dsilu, grid_for((size_t)n * kWidth), dim3(256), 0, s, m->dY, kWidth, pre2,
-                     m->dpb, (size_t)n, kWidth);
-  // layer 2
-  gemm(s, true, false, true, act1, kWidth, m->dpb, kWidth, G(m, w1 + 2), kWidth, kWidth, kWidth, n,
-       1.f);
-  hipLaunchKernelGGL((k_colsum<uint16_t>), dim3((n + 511) / 512), dim3(kWidth), 0, s, m->dpb,
-                     kWidth, (size_t)n, kWidth, nullptr, G(m, w1 + 3));
-  gemm(s, false, true, false, m->dpb, kWidth, W(m, w1 + 2), kWidth, m->dY, kWidth, n, kWidth,
-       kWidth, 1.f);
-  hipLaunchKernelGGL(k_dsilu, grid_for((size_t)n * kWidth), dim3(256), 0, s, m->dY, kWidth, pre1,
-                     m->dpb, (size_t)n, kWidth);
-  // layer 1
-  gemm(s, true, false, true, u, kin, m->dpb, kWidth, G(m, w1), kWidth, kin, kWidth, n, 1.f);
-  hipLaunchKernelGGL((k_colsum<uint16_t>), dim3((n + 511) / 512), dim3(kWidth), 0, s, m->dpb,
-                     kWidth, (size_t)n, kWidth, nullptr, G(m, w1 + 1));
-  gemm(s, false, true, false, m->dpb, kWidth, W(m, w1), kWidth, m->dY, kIn2, n, kin_used, kWidth,
-       1.f);
-}
-
-// Backward of an edge round given dM (f32, stride ldm): gradients of W3/b3,
-// W2/b2, W1/b1; leaves dz1 (bf16, e x 128) in m->dz1.
-static void edge_back(clsph_nn* m, int r, int w1, int w2, int w3, const float* dM, int ldm,
-                      hipStream_t s) {
-  int n = (int)m->n, e = (int)m->e;
-  size_t ew = (size_t)e * kWidth;
-  // bf16(dM) into dpb (n x 128)
-  hipLaunchKernelGGL(k_cast_2d, grid_for((size_t)n * kWidth), dim3(256), 0, s, dM, ldm, m->dpb,
-                     kWidth, (size_t)n, kWidth);
-  gemm(s, true, false, true, m->S[r], kWidth, m->dpb, kWidth, G(m, w3), kWidth, kWidth, kWidth, n,
-       1.f);
-  hipLaunchKernelGGL((k_colsum<float>), dim3((n + 511) / 512), dim3(kWidth), 0, s, dM, ldm,
-                     (size_t)n, kWidth, m->degc, G(m, w3 + 1));
-  gemm(s, false, true, false, m->dpb, kWidth, W(m, w3), kWidth, m->dS, kWidth, n, kWidth, kWidth,
-       1.f / kAggC);
-  // edges
-  hipLaunchKernelGGL(k_edge_dz, grid_for(ew), dim3(256), 0, s, m->dS, m->z2[r], m->dst, m->dz2,
-                     (size_t)e);
-  gemm(s, true, false, true, m->s1[r], kWidth, m->dz2, kWidth, G(m, w2), kWidth, kWidth, kWidth, e,
-       1.f);
-  hipLaunchKernelGGL((k_colsum<uint16_t>), dim3((e + 511) / 512), dim3(kWidth), 0, s, m->dz2,
-                     kWidth, (size_t)e, kWidth, nullptr, G(m, w2 + 1));
-  gemm(s, false, true, false, m->dz2, kWidth, W(m, w2), kWidth, m->ds1, kWidth, e, kWidth, kWidth,
-       1.f);
-  hipLaunchKernelGGL(k_dsilu, grid_for(ew), dim3(256), 0, s, m->ds1, kWidth, m->z1[r], m->dz1,
-                     (size_t)e, kWidth);
-  gemm(s, true, false, true, m->E1, kFeat, m->dz1, kWidth, G(m, w1), kWidth, kFeat, kWidth, e,
-       1.f);
-  hipLaunchKernelGGL((k_colsum<uint16_t>), dim3((e + 511) / 512), dim3(kWidth), 0, s, m->dz1,
-                     kWidth, (size_t)e, kWidth, nullptr, G(m, w1 + 1));
-}
-
+// Backward of the node MLP and the edge layer.
 void backward(clsph_nn* m) {
   hipStream_t s = m->ctx->stream;
-  int n = (int)m->n;
-  size_t nw = (size_t)n * kWidth;
+  int n = (int)m->n, e = (int)m->e;
+  size_t nw = (size_t)n * kNodeW;
   fill(m->grad, kParamTotal, 0.f, s);
   fill(m->loss, 1, 0.f, s);
   hipLaunchKernelGGL(k_loss, grid_for(n), dim3(256), 0, s, m->out, m->target, m->n, m->dout,
                      m->loss);
-  // dpb (n x 256 bf16): the first half is mlp3_back's scratch, the second
-  // half holds the bf16 gradient fed into it.
-  uint16_t* dy_b = m->dpb + nw;
-  hipLaunchKernelGGL(k_cast, grid_for((size_t)n * kOut), dim3(256), 0, s, m->dout, dy_b,
+  hipLaunchKernelGGL(k_cast, grid_for((size_t)n * kOut), dim3(256), 0, s, m->dout, m->dout_b,
                      (size_t)n * kOut);
-  // node MLP 2: dY (stride 256) = [d H | d M2]
-  mlp3_back(m, N2_W1, m->u2, kIn2, m->q1, m->w1, m->q2, m->w2, dy_b, m->dout, kOut, kOut, kIn2,
-            s);
-  hipLaunchKernelGGL(k_copy_2d, grid_for(nw), dim3(256), 0, s, m->dY, kIn2, m->dH, kWidth,
-                     (size_t)n, kWidth);
-  // edge round 2 and its node terms
-  edge_back(m, 1, E2_W1, E2_W2, E2_W3, m->dY + kWidth, kIn2, s);
-  hipLaunchKernelGGL(k_segsum_rows, dim3((n + 3) / 4), dim3(256), 0, s, m->dz1, m->off, m->n,
-                     m->dPA);
-  fill(m->dPB, nw, 0.f, s);
-  hipLaunchKernelGGL(k_scatter_rows, grid_for((size_t)m->e * (kWidth / 2)), dim3(256), 0, s,
-                     m->dz1, m->src, (size_t)m->e, m->dPB);
-  hipLaunchKernelGGL(k_cast, grid_for(nw), dim3(256), 0, s, m->dPA, m->dPA_b, nw);
-  hipLaunchKernelGGL(k_cast, grid_for(nw), dim3(256), 0, s, m->dPB, m->dPB_b, nw);
-  gemm(s, true, false, true, m->Hb, kWidth, m->dPA_b, kWidth, G(m, E2_WA), kWidth, kWidth, kWidth,
-       n, 1.f);
-  gemm(s, true, false, true, m->Hb, kWidth, m->dPB_b, kWidth, G(m, E2_WB), kWidth, kWidth, kWidth,
-       n, 1.f);
-  gemm(s, false, true, true, m->dPA_b, kWidth, W(m, E2_WA), kWidth, m->dH, kWidth, n, kWidth,
-       kWidth, 1.f);
-  gemm(s, false, true, true, m->dPB_b, kWidth, W(m, E2_WB), kWidth, m->dH, kWidth, n, kWidth,
-       kWidth, 1.f);
-  // node MLP 1: dY[:, :128] = d M1
-  hipLaunchKernelGGL(k_cast, grid_for(nw), dim3(256), 0, s, m->dH, dy_b, nw);
-  mlp3_back(m, N1_W1, m->u1, kIn1, m->p1, m->y1, m->p2, m->y2, dy_b, m->dH, kWidth, kWidth,
-            kWidth, s);
-  // edge round 1
-  edge_back(m, 0, E1_W1, E1_W2, E1_W3, m->dY, kIn2, s);
+  const dim3 nchunks((n + 511) / 512);
+  // layer 3
+  gemm(s, true, false, true, m->y2, kNodeW, m->dout_b, kOut, G(m, N_W3), kOut, kNodeW, kOut, n,
+       1.f);
+  hipLaunchKernelGGL((k_colsum<float>), nchunks, dim3(kOut), 0, s, m->dout, kOut, (size_t)n, kOut,
+                     G(m, N_B3));
+  gemm(s, false, true, false, m->dout_b, kOut, W(m, N_W3), kOut, m->dY, kNodeW, n, kNodeW, kOut,
+       1.f);
+  hipLaunchKernelGGL(k_dsilu, grid_for(nw), dim3(256), 0, s, m->dY, kNodeW, m->p2, m->dpb,
+                     (size_t)n, kNodeW);
+  // layer 2
+  gemm(s, true, false, true, m->y1, kNodeW, m->dpb, kNodeW, G(m, N_W2), kNodeW, kNodeW, kNodeW, n,
+       1.f);
+  hipLaunchKernelGGL((k_colsum<uint16_t>), nchunks, dim3(kNodeW), 0, s, m->dpb, kNodeW, (size_t)n,
+                     kNodeW, G(m, N_B2));
+  gemm(s, false, true, false, m->dpb, kNodeW, W(m, N_W2), kNodeW, m->dY, kNodeW, n, kNodeW,
+       kNodeW, 1.f);
+  hipLaunchKernelGGL(k_dsilu, grid_for(nw), dim3(256), 0, s, m->dY, kNodeW, m->p1, m->dpb,
+                     (size_t)n, kNodeW);
+  // layer 1, and d loss / d S (the first 32 inputs) into dY
+  gemm(s, true, false, true, m->u, kIn, m->dpb, kNodeW, G(m, N_W1), kNodeW, kIn, kNodeW, n, 1.f);
+  hipLaunchKernelGGL((k_colsum<uint16_t>), nchunks, dim3(kNodeW), 0, s, m->dpb, kNodeW, (size_t)n,
+                     kNodeW, G(m, N_B1));
+  gemm(s, false, true, false, m->dpb, kNodeW, W(m, N_W1), kNodeW, m->dY, kNodeW, n, kEdgeW,
+       kNodeW, 1.f);
+  // edge layer
+  hipLaunchKernelGGL(k_edge_dz, grid_for((size_t)e * kEdgeW), dim3(256), 0, s, m->dY, kNodeW,
+                     m->z, m->dst, m->dz, (size_t)e);
+  gemm(s, true, false, true, m->E1, kFeat, m->dz, kEdgeW, G(m, E_W1), kEdgeW, kFeat, kEdgeW, e,
+       1.f);
+  hipLaunchKernelGGL((k_colsum<uint16_t>), dim3((e + 511) / 512), dim3(kEdgeW), 0, s, m->dz,
+                     kEdgeW, (size_t)e, kEdgeW, G(m, E_B1));
 }
 
 void adam(clsph_nn* m, float lr, float beta1, float beta2, float eps) {
@@ -1360,12 +958,10 @@ void cast_params(clsph_nn* m) {
                      m->wb, (size_t)kParamTotal);
 }
 
-void launch_target_mean(clsph_nn* m, clsph_state* learner, clsph_state* teacher, uint32_t nsub,
-                        float gx, float gy, float gz) {
+void launch_target_mean(clsph_nn* m, clsph_state* learner, clsph_state* teacher, uint32_t nsub) {
   hipStream_t s = m->ctx->stream;
   uint32_t n = learner->n;
-  float* sum = m->tmp;  // 3 n floats
-  fill(sum, 3 * (size_t)n, 0.f, s);
+  fill(m->sum, 3 * (size_t)n, 0.f, s);
   for (uint32_t k = 0; k < nsub; ++k) {
     launch_grid_params(teacher);
     launch_hash(teacher);
@@ -1375,21 +971,20 @@ void launch_target_mean(clsph_nn* m, clsph_state* learner, clsph_state* teacher,
     launch_density(teacher);
     launch_forces(teacher);
     hipLaunchKernelGGL(k_accum_acc, grid_for(n), dim3(256), 0, s, teacher->acc_s, teacher->val_b,
-                       teacher->ids, n, sum);
+                       teacher->ids, n, m->sum);
     launch_advect(teacher);
     launch_track_ids(teacher);
   }
-  hipLaunchKernelGGL(k_target_mean, grid_for(n), dim3(256), 0, s, learner->val_b, sum, n,
-                     1.f / (float)nsub, gx, gy, gz, m->target);
+  hipLaunchKernelGGL(k_target_mean, grid_for(n), dim3(256), 0, s, learner->val_b, m->sum,
+                     learner->acc_s, n, 1.f / (float)nsub, m->target);
 }
 
-void launch_target(clsph_nn* m, clsph_state* a, clsph_state* t, float dt, float gx, float gy,
-                   float gz) {
+void launch_target(clsph_nn* m, clsph_state* a, clsph_state* t, float dt) {
   hipStream_t s = m->ctx->stream;
   uint32_t n = a->n;
   hipLaunchKernelGGL(k_invert, grid_for(n), dim3(256), 0, s, t->ids, n, m->loc);
-  hipLaunchKernelGGL(k_target, grid_for(n), dim3(256), 0, s, a->val_b, a->vh, m->loc, t->vh, n,
-                     1.f / dt, gx, gy, gz, m->target);
+  hipLaunchKernelGGL(k_target, grid_for(n), dim3(256), 0, s, a->val_b, a->vh, m->loc, t->vh,
+                     a->acc_s, n, 1.f / dt, m->target);
 }
 
 void gemm_public(hipStream_t s, bool ta, bool tb, bool add, const uint16_t* A, int lda,

@@ -18,14 +18,10 @@ namespace {
 void free_nn(clsph_nn* m) {
   dfree(m->params), dfree(m->wb), dfree(m->grad), dfree(m->adam_m), dfree(m->adam_v);
   dfree(m->off), dfree(m->src), dfree(m->dst), dfree(m->scan_part);
-  dfree(m->E1), dfree(m->nu), dfree(m->degc);
-  for (int r = 0; r < 2; ++r) dfree(m->z1[r]), dfree(m->s1[r]), dfree(m->z2[r]), dfree(m->S[r]);
-  dfree(m->tmp), dfree(m->u1), dfree(m->p1), dfree(m->y1), dfree(m->p2), dfree(m->y2);
-  dfree(m->Hb), dfree(m->PA), dfree(m->PB), dfree(m->u2), dfree(m->q1), dfree(m->w1);
-  dfree(m->q2), dfree(m->w2), dfree(m->out);
-  dfree(m->target), dfree(m->loss), dfree(m->dout), dfree(m->dY), dfree(m->dpb), dfree(m->dS);
-  dfree(m->dz2), dfree(m->ds1), dfree(m->dz1), dfree(m->dPA), dfree(m->dPB), dfree(m->dH);
-  dfree(m->dPA_b), dfree(m->dPB_b), dfree(m->loc), dfree(m->inv), dfree(m->cmp);
+  dfree(m->E1), dfree(m->nu), dfree(m->degc), dfree(m->z), dfree(m->S), dfree(m->u);
+  dfree(m->p1), dfree(m->y1), dfree(m->p2), dfree(m->y2), dfree(m->out);
+  dfree(m->target), dfree(m->loss), dfree(m->dout), dfree(m->dout_b), dfree(m->dY);
+  dfree(m->dpb), dfree(m->dz), dfree(m->sum), dfree(m->loc), dfree(m->inv), dfree(m->cmp);
 }
 
 int check_state(clsph_nn* m, clsph_state* st) {
@@ -77,7 +73,8 @@ int clsph_nn_create(clsph_ctx* ctx, uint32_t n_cap, uint32_t e_cap, clsph_nn** o
   m->ctx = ctx;
   m->n_cap = n_cap;
   m->e_cap = e_cap ? e_cap : 64u * n_cap;
-  size_t N = n_cap, E = m->e_cap, Wd = nn::kWidth, P = nn::kParamTotal;
+  size_t N = n_cap, E = m->e_cap, P = nn::kParamTotal;
+  size_t EW = nn::kEdgeW, NW = nn::kNodeW;
   int rc = CLSPH_OK;
 #define AL(ptr, count) \
   if (rc == CLSPH_OK) rc = dalloc(ctx, &(ptr), (count))
@@ -85,17 +82,12 @@ int clsph_nn_create(clsph_ctx* ctx, uint32_t n_cap, uint32_t e_cap, clsph_nn** o
   AL(m->off, N + 1); AL(m->src, E); AL(m->dst, E);
   AL(m->scan_part, N / 1024 + 2);
   AL(m->E1, E * nn::kFeat); AL(m->nu, 2 * N); AL(m->degc, N);
-  for (int r = 0; r < 2; ++r) {
-    AL(m->z1[r], E * Wd); AL(m->s1[r], E * Wd); AL(m->z2[r], E * Wd); AL(m->S[r], N * Wd);
-  }
-  AL(m->tmp, N * nn::kIn2); AL(m->u1, N * nn::kIn1); AL(m->p1, N * Wd); AL(m->y1, N * Wd);
-  AL(m->p2, N * Wd); AL(m->y2, N * Wd); AL(m->Hb, N * Wd); AL(m->PA, N * Wd); AL(m->PB, N * Wd);
-  AL(m->u2, N * nn::kIn2); AL(m->q1, N * Wd); AL(m->w1, N * Wd); AL(m->q2, N * Wd);
-  AL(m->w2, N * Wd); AL(m->out, N * nn::kOut);
-  AL(m->target, 3 * N); AL(m->loss, 1); AL(m->dout, N * nn::kOut); AL(m->dY, N * nn::kIn2);
-  AL(m->dpb, N * nn::kIn2); AL(m->dS, N * Wd); AL(m->dz2, E * Wd); AL(m->ds1, E * Wd);
-  AL(m->dz1, E * Wd); AL(m->dPA, N * Wd); AL(m->dPB, N * Wd); AL(m->dH, N * Wd);
-  AL(m->dPA_b, N * Wd); AL(m->dPB_b, N * Wd); AL(m->loc, N); AL(m->inv, N); AL(m->cmp, 2);
+  AL(m->z, E * EW); AL(m->S, N * EW); AL(m->u, N * nn::kIn);
+  AL(m->p1, N * NW); AL(m->y1, N * NW); AL(m->p2, N * NW); AL(m->y2, N * NW);
+  AL(m->out, N * nn::kOut);
+  AL(m->target, 3 * N); AL(m->loss, 1); AL(m->dout, N * nn::kOut); AL(m->dout_b, N * nn::kOut);
+  AL(m->dY, N * NW); AL(m->dpb, N * NW); AL(m->dz, E * EW); AL(m->sum, 3 * N);
+  AL(m->loc, N); AL(m->inv, N); AL(m->cmp, 2);
 #undef AL
   if (rc) {
     free_nn(m);
@@ -153,16 +145,15 @@ int clsph_nn_get_params(clsph_nn* m, float* params, uint32_t count) {
 int clsph_nn_forward(clsph_nn* m, clsph_state* st, uint32_t* edges) {
   int rc = check_state(m, st);
   if (rc) return rc;
-  if (st->stage != kStageDensity && st->stage != kStageForces)
-    return fail(m->ctx, CLSPH_EINVAL, "clsph_nn_forward needs the density stage of the substep");
+  if (st->stage != kStageForces || st->acc_learned)
+    return fail(m->ctx, CLSPH_EINVAL, "clsph_nn_forward needs the forces stage of the substep");
   rc = build_graph(m, st);
   if (rc) return rc;
   float g[3];
   gravity(st, g);
-  nn::forward(m, nullptr, st->acc_s, g[0], g[1], g[2]);
+  nn::forward(m, st->acc_s, g[0], g[1], g[2]);
   rc = check_launch(m->ctx);
   if (rc) return rc;
-  st->stage = kStageForces;
   if (edges) *edges = m->e;
   return CLSPH_OK;
 }
@@ -170,14 +161,15 @@ int clsph_nn_forward(clsph_nn* m, clsph_state* st, uint32_t* edges) {
 int clsph_nn_infer(clsph_nn* m, clsph_state* st) {
   int rc = check_state(m, st);
   if (rc) return rc;
-  if (st->stage != kStageDensity && st->stage != kStageForces)
-    return fail(m->ctx, CLSPH_EINVAL, "clsph_nn_infer needs the density stage of the substep");
+  if (st->stage != kStageForces || st->acc_learned)
+    return fail(m->ctx, CLSPH_EINVAL,
+                "clsph_nn_infer needs the forces stage of the substep, once");
   float g[3];
   gravity(st, g);
-  nn::forward_fused(m, st, st->acc_s, g[0], g[1], g[2]);
+  nn::forward_fused(m, st, st->acc_s, m->out, g[0], g[1], g[2]);
   rc = check_launch(m->ctx);
   if (rc) return rc;
-  st->stage = kStageForces;
+  st->acc_learned = true;
   return CLSPH_OK;
 }
 
@@ -186,31 +178,32 @@ int clsph_nn_profile(clsph_nn* m, clsph_state* st, uint32_t reps, float* ms,
   int rc = check_state(m, st);
   if (rc) return rc;
   if (!ms || reps == 0) return fail(m->ctx, CLSPH_EINVAL, "clsph_nn_profile: bad argument");
-  if (st->stage != kStageDensity && st->stage != kStageForces)
-    return fail(m->ctx, CLSPH_EINVAL, "clsph_nn_profile needs the density stage of the substep");
+  if (st->stage != kStageForces)
+    return fail(m->ctx, CLSPH_EINVAL, "clsph_nn_profile needs the forces stage of the substep");
   float g[3];
   gravity(st, g);
-  hipEvent_t ev[5];
+  // the accelerations are copied into dY (n x 64 floats holds n float4s)
+  float4* acc = reinterpret_cast<float4*>(m->dY);
+  HIPCK(m->ctx, hipMemcpyAsync(acc, st->acc_s, sizeof(float4) * st->n, hipMemcpyDeviceToDevice,
+                               m->ctx->stream));
+  hipEvent_t ev[2];
   for (auto& e : ev) HIPCK(m->ctx, hipEventCreate(&e));
-  double acc[4] = {0, 0, 0, 0};
+  double total = 0.0;
   for (uint32_t r = 0; r < reps && !rc; ++r) {
-    nn::forward_fused(m, st, st->acc_s, g[0], g[1], g[2], ev);
+    nn::forward_fused(m, st, acc, nullptr, g[0], g[1], g[2], ev);
     rc = check_launch(m->ctx);
     if (rc) break;
-    if (hipEventSynchronize(ev[4]) != hipSuccess) {
+    if (hipEventSynchronize(ev[1]) != hipSuccess) {
       rc = fail(m->ctx, CLSPH_EDEVICE, "clsph_nn_profile: event wait failed");
       break;
     }
-    for (int k = 0; k < 4; ++k) {
-      float t = 0.f;
-      (void)hipEventElapsedTime(&t, ev[k], ev[k + 1]);
-      acc[k] += t;
-    }
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, ev[0], ev[1]);
+    total += t;
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   if (rc) return rc;
-  for (int k = 0; k < 4; ++k) ms[k] = (float)(acc[k] / reps);
-  st->stage = kStageForces;
+  ms[0] = (float)(total / reps);
   if (edges) {
     std::vector<uint32_t> cnt(st->n);
     HIPCK(m->ctx, hipMemcpy(cnt.data(), st->nbr_count, sizeof(uint32_t) * st->n,
@@ -265,11 +258,10 @@ int clsph_nn_target_from(clsph_nn* m, clsph_state* learner, clsph_state* teacher
   if (!teacher || teacher->n != learner->n || !teacher->ids)
     return fail(m->ctx, CLSPH_EINVAL,
                 "the teacher state must hold the learner's particles and track identities");
-  if (learner->stage < kStageSort || !(dt > 0.f))
-    return fail(m->ctx, CLSPH_EINVAL, "the learner must be sorted and dt positive");
-  float g[3];
-  gravity(learner, g);
-  nn::launch_target(m, learner, teacher, dt, g[0], g[1], g[2]);
+  if (learner->stage != kStageForces || learner->acc_learned || !(dt > 0.f))
+    return fail(m->ctx, CLSPH_EINVAL,
+                "the learner must have run its forces stage and dt must be positive");
+  nn::launch_target(m, learner, teacher, dt);
   return check_launch(m->ctx);
 }
 
@@ -280,11 +272,10 @@ int clsph_nn_target_mean_acc(clsph_nn* m, clsph_state* learner, clsph_state* tea
   if (!teacher || teacher->n != learner->n || !teacher->ids || teacher->n_scenes != 1)
     return fail(m->ctx, CLSPH_EINVAL,
                 "the teacher state must hold the learner's particles and track identities");
-  if (learner->stage < kStageSort || nsub == 0)
-    return fail(m->ctx, CLSPH_EINVAL, "the learner must be sorted and nsub positive");
-  float g[3];
-  gravity(learner, g);
-  nn::launch_target_mean(m, learner, teacher, nsub, g[0], g[1], g[2]);
+  if (learner->stage != kStageForces || learner->acc_learned || nsub == 0)
+    return fail(m->ctx, CLSPH_EINVAL,
+                "the learner must have run its forces stage and nsub must be positive");
+  nn::launch_target_mean(m, learner, teacher, nsub);
   rc = check_launch(m->ctx);
   if (rc) return rc;
   teacher->acc_zero = true;
@@ -400,12 +391,14 @@ int clsph_learned_step(clsph_state* st, clsph_nn* m, uint32_t nsteps) {
     launch_permute(st);
     launch_cell_table(st);
     launch_density(st);
-    nn::forward_fused(m, st, st->acc_s, g[0], g[1], g[2]);
+    launch_forces(st);
+    nn::forward_fused(m, st, st->acc_s, nullptr, g[0], g[1], g[2]);
     launch_advect(st);
     if (st->ids) launch_track_ids(st);
     rc = check_launch(st->ctx);
     if (rc) return rc;
     st->acc_zero = true;
+    st->acc_learned = false;
     st->fields_canonical = false;
     st->stage = kStageAdvect;
   }

@@ -109,6 +109,9 @@ struct clsph_state {
   // acceleration is the zero vector advection leaves behind.
   int stage = -1;
   bool acc_zero = false;
+  // True once the learned surrogate has added its correction to the forces
+  // stage's accelerations (cleared by every stage).
+  bool acc_learned = false;
   // True while rho_s / prs_s / key_b / acc_s still hold the uploaded
   // particles in canonical order (no substep has run since the upload).
   bool fields_canonical = true;

@@ -1,25 +1,27 @@
 """Training and running the learned force surrogate.
 
-The surrogate (the network of libclsph_amd/csrc/nn.h) predicts each
-particle's acceleration over one macro step of 4 substeps, so that the
-learned method advances at dt_macro = 4 dt with one neighbour search, one
-network evaluation and one advection per macro step.
+The surrogate (the network of libclsph_amd/csrc/nn.h) corrects each
+particle's forces-stage acceleration a0 so that it stands for the mean
+acceleration over one macro step of 4 substeps: the learned method advances
+at dt_macro = 4 dt with one plain substep per macro step plus one network
+evaluation, a = a0 + R out.
 
 Training data comes from the plain GPU solver, generated online: each
 trajectory is a seeded dam-break variant, advanced by the plain substep;
 every `gap` substeps the current state is copied into a learner state,
-sorted and given its densities, and the teacher's next 4 plain substeps
-(with particle identities tracked) give each particle's target
+which runs the plain substep's stages up to forces (a0), and the teacher's
+next 4 plain substeps (with particle identities tracked) give each
+particle's target
 
-    a*_i = mean over the 4 substeps of the forces stage's a_i
+    out*_i = (mean over the 4 substeps of the forces stage's a_i - a0_i) / R
 
 (gravity plus pressure, viscosity and surface tension along the plain
 trajectory; collision impulses stay with advection, which applies them in
 the learned step too). Each sample is one forward, one backward and one
 Adam step, all on the GPU.
 
-Running (run_learned): macro steps with the network in place of the forces
-stage. Every `judge_every` macro steps the plain method is the judge: the
+Running (run_learned): macro steps of the plain substep at 4 dt with the
+network's correction. Every `judge_every` macro steps the plain method is the judge: the
 same start is advanced by 4 plain substeps, both states get their densities,
 and when max_i |rho_i - rho_i^plain| / rho0 exceeds `tol` (1 %), the macro
 step is replaced by the plain result (a fallback).
@@ -41,19 +43,16 @@ SPACING_1M = (0.128 / 1048576) ** (1.0 / 3.0)
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",
                        "surrogate_mi355x.npy")
 DENSITY_STAGES = ("grid", "hash", "sort", "cell_table", "density")
+FORCES_STAGES = DENSITY_STAGES + ("forces",)
 
 # Parameter vector layout of the network (libclsph_amd/csrc/nn.h, enum Param):
 # (name, shape, used input rows) in storage order, row-major inputs x outputs.
-W, FEAT, IN1, IN2, OUT = 128, 32, 144, 256, 32
+EW, NW, FEAT, IN, OUT = 32, 64, 32, 64, 32
 LAYOUT = [
-    ("e1.w1", (FEAT, W), 22), ("e1.b1", (W,), 0), ("e1.w2", (W, W), W), ("e1.b2", (W,), 0),
-    ("e1.w3", (W, W), W), ("e1.b3", (W,), 0),
-    ("n1.w1", (IN1, W), W + 2), ("n1.b1", (W,), 0), ("n1.w2", (W, W), W), ("n1.b2", (W,), 0),
-    ("n1.w3", (W, W), W), ("n1.b3", (W,), 0),
-    ("e2.w1", (FEAT, W), 22), ("e2.b1", (W,), 0), ("e2.wa", (W, W), W), ("e2.wb", (W, W), W),
-    ("e2.w2", (W, W), W), ("e2.b2", (W,), 0), ("e2.w3", (W, W), W), ("e2.b3", (W,), 0),
-    ("n2.w1", (IN2, W), IN2), ("n2.b1", (W,), 0), ("n2.w2", (W, W), W), ("n2.b2", (W,), 0),
-    ("n2.w3", (W, OUT), W), ("n2.b3", (OUT,), 0),
+    ("e.w1", (FEAT, EW), 22), ("e.b1", (EW,), 0),
+    ("n.w1", (IN, NW), 38), ("n.b1", (NW,), 0),
+    ("n.w2", (NW, NW), NW), ("n.b2", (NW,), 0),
+    ("n.w3", (NW, OUT), NW), ("n.b3", (OUT,), 0),
 ]
 
 
@@ -69,7 +68,7 @@ def init_params(seed=0):
             continue
         w = rng.uniform(-1.0, 1.0, size=shape) / np.sqrt(used)
         w[used:] = 0.0
-        if name == "n2.w3":
+        if name == "n.w3":
             w[:, 3:] = 0.0
             w *= 0.1
         parts.append(w)
@@ -119,6 +118,11 @@ def density_stages(st):
         st.stage(name)
 
 
+def forces_stages(st):
+    for name in FORCES_STAGES:
+        st.stage(name)
+
+
 def train(net, ctx, trajectories=200, steps=2000, gap=20, minutes=None, lr=1e-3, lr_end=1e-5,
           seed=0, log=print, warm=50):
     """Online training over plain-solver trajectories; returns the loss
@@ -138,7 +142,7 @@ def train(net, ctx, trajectories=200, steps=2000, gap=20, minutes=None, lr=1e-3,
         teacher.step(warm)
         for smp in range(samples_per_traj):
             learner.copy_from(teacher)
-            density_stages(learner)
+            forces_stages(learner)
             teacher.track_ids(True)
             net.target_mean_acc(learner, teacher, MACRO)  # advances the teacher by MACRO
             try:

@@ -398,26 +398,27 @@ class Network:
         return out
 
     def forward(self, state):
-        """Evaluate on a state whose last stage was density; returns the
-        edge count."""
+        """Training evaluation on a state whose last stage was forces (the
+        state is not changed); returns the edge count."""
         e = np.zeros(1, dtype=np.uint32)
         self.ctx.check(lib.clsph_nn_forward(self.h, state.h, _ptr(e)))
         self.n, self.e = state.n, int(e[0])
         return self.e
 
     def infer(self, state):
-        """Evaluate with the fused inference kernels (no gradients)."""
+        """Fused inference on a state whose last stage was forces: adds the
+        correction to its accelerations (once per forces stage)."""
         self.ctx.check(lib.clsph_nn_infer(self.h, state.h))
         self.n = state.n
 
     def profile(self, state, reps=5):
-        """Average ms of (edge round 1, node round 1, edge round 2, node
-        round 2) over `reps` fused evaluations, and the edges per pass."""
-        ms = np.zeros(4, dtype=np.float32)
+        """Average ms of one fused evaluation over `reps` runs (on a copy of
+        the state's accelerations), and the edges per pass."""
+        ms = np.zeros(1, dtype=np.float32)
         e = np.zeros(1, dtype=np.uint64)
         self.ctx.check(lib.clsph_nn_profile(self.h, state.h, reps, _ptr(ms), _ptr(e)))
         self.n = state.n
-        return [float(v) for v in ms], int(e[0])
+        return float(ms[0]), int(e[0])
 
     def graph(self):
         off = np.zeros(self.n + 1, dtype=np.uint32)
@@ -445,8 +446,8 @@ class Network:
         self.ctx.check(lib.clsph_nn_target_from(self.h, learner.h, teacher.h, ctypes.c_float(dt)))
 
     def target_mean_acc(self, learner, teacher, nsub):
-        """Target: mean forces-stage acceleration over nsub plain substeps of
-        `teacher` (which they advance)."""
+        """Target: (mean forces-stage acceleration over nsub plain substeps of
+        `teacher`, which they advance, minus the learner's own) / 2."""
         self.ctx.check(lib.clsph_nn_target_mean_acc(self.h, learner.h, teacher.h, nsub))
         self.n = learner.n
 

@@ -5,9 +5,11 @@ module; the product path never calls it.
 
 The surrogate has no counterpart in the reference: the reference predicts
 accelerations with the `forces` kernel only (kernels/sph.cl:42-62). This file
-is the definition the GPU kernels are checked against. It is a
-message-passing network over the particles' neighbour graph (r < h, self
-excluded, at most 128 neighbours per particle) with two rounds:
+is the definition the GPU kernels are checked against. The network corrects
+the forces stage's acceleration a0_i so that it stands for the mean
+acceleration over a macro step of 4 substeps. It is one round of message
+passing over the particles' neighbour graph (r < h, self excluded, at most
+128 neighbours per particle):
 
   node inputs   nu_i = [rho_i / rho0 - 1, asinh(p_i / K) / 4]
   edge inputs   e_ij = [(x_i - x_j) / h, (v_i - v_j) / V, q, (1 - q^2)^3,
@@ -18,54 +20,42 @@ excluded, at most 128 neighbours per particle) with two rounds:
                 the pair terms are those the forces kernel sums
                 (kernels/forces.cl:60-101), from positions, velocities,
                 densities and pressures only
-  round 1       s1 = silu(e W1 + b1); s2 = silu(s1 W2 + b2)
-                M1_i = (sum_j s2_ij / C) W3 + (deg_i / C) b3
-                H_i  = MLP_n1([M1_i, nu_i, 0...])        (3 layers, 144 -> 128)
-  round 2       z1 = e W1' + b1' + (H_i Wa) + (H_j Wb)
-                s1 = silu(z1); s2 = silu(s1 W2' + b2')
-                M2_i = (sum_j s2_ij / C) W3' + (deg_i / C) b3'
-                out_i = MLP_n2([H_i, M2_i])              (3 layers, 256 -> 32)
-  acceleration  a_i = g + A * out_i[0:3]
+  edges         S_i = bf16(sum_j relu(e_ij W1 + b1) / C)              (32 wide)
+  node          u_i = [S_i, nu_i, (a0_i - g) / A, deg_i / C, 0 x 26]  (64 wide)
+                out_i = MLP(u_i): 64 -> 64 -> 64 -> 32, SiLU between layers
+  acceleration  a_i = a0_i + R * out_i[0:3]
 
-The third edge layer is applied after the sum over neighbours. That is exact,
-because the layer is linear: sum_j (s2_ij W3 + b3) = (sum_j s2_ij) W3 + deg_i b3.
-The identity is pinned in the tests against the literal per-edge form.
+With every parameter zero the network returns a0 exactly, so the method
+without it is the plain substep at the macro step.
 
 Rounding points. The GPU rounds every matrix-product operand to bfloat16
-(round to nearest even) and accumulates in float32. `forward(..., bf16=True)`
-rounds at the same points, so the GPU forward can be checked tightly;
-`bf16=False` is the exact FP64 network. `backward` is the analytic gradient
-of the FP64 network (rounding treated as the identity). It is pinned by
-central finite differences of `forward`.
+(round to nearest even) and accumulates in float32. `forward(...,
+bf16_round=True)` rounds at the same points, so the GPU forward can be
+checked tightly; `bf16_round=False` is the exact FP64 network. `backward`
+is the analytic gradient of the FP64 network (rounding treated as the
+identity). It is pinned by central finite differences of `forward`.
 """
 import numpy as np
 
-WIDTH = 128
+EDGE_W = 32        # edge layer width
+NODE_W = 64        # node MLP width
 FEAT = 32          # edge input features (22 used, 10 zero)
 FEAT_USED = 22
-NODE_IN1 = 144     # node MLP 1 input: M1 (128) + nu (2) + zeros
-NODE_IN2 = 256     # node MLP 2 input: H (128) + M2 (128)
-OUT = 32           # node MLP 2 output (3 used)
+NODE_IN = 64       # node MLP input: S (32) + nu (2) + a0 (3) + deg (1) + zeros
+NODE_IN_USED = 38
+OUT = 32           # network output (3 used)
 AGG_C = 32.0       # aggregation normaliser
 VEL_SCALE = 1.0    # V, m/s
 ACC_SCALE = 10.0   # A, m/s^2
+RES_SCALE = 2.0    # R, m/s^2
 MAX_EDGES = 128
 
 # Flat parameter vector: (name, shape) in storage order (row-major, in x out).
 LAYOUT = [
-    ("e1.w1", (FEAT, WIDTH)), ("e1.b1", (WIDTH,)),
-    ("e1.w2", (WIDTH, WIDTH)), ("e1.b2", (WIDTH,)),
-    ("e1.w3", (WIDTH, WIDTH)), ("e1.b3", (WIDTH,)),
-    ("n1.w1", (NODE_IN1, WIDTH)), ("n1.b1", (WIDTH,)),
-    ("n1.w2", (WIDTH, WIDTH)), ("n1.b2", (WIDTH,)),
-    ("n1.w3", (WIDTH, WIDTH)), ("n1.b3", (WIDTH,)),
-    ("e2.w1", (FEAT, WIDTH)), ("e2.b1", (WIDTH,)),
-    ("e2.wa", (WIDTH, WIDTH)), ("e2.wb", (WIDTH, WIDTH)),
-    ("e2.w2", (WIDTH, WIDTH)), ("e2.b2", (WIDTH,)),
-    ("e2.w3", (WIDTH, WIDTH)), ("e2.b3", (WIDTH,)),
-    ("n2.w1", (NODE_IN2, WIDTH)), ("n2.b1", (WIDTH,)),
-    ("n2.w2", (WIDTH, WIDTH)), ("n2.b2", (WIDTH,)),
-    ("n2.w3", (WIDTH, OUT)), ("n2.b3", (OUT,)),
+    ("e.w1", (FEAT, EDGE_W)), ("e.b1", (EDGE_W,)),
+    ("n.w1", (NODE_IN, NODE_W)), ("n.b1", (NODE_W,)),
+    ("n.w2", (NODE_W, NODE_W)), ("n.b2", (NODE_W,)),
+    ("n.w3", (NODE_W, OUT)), ("n.b3", (OUT,)),
 ]
 
 
@@ -101,6 +91,10 @@ def silu(x):
 def dsilu(x):
     s = 1.0 / (1.0 + np.exp(-x))
     return s * (1.0 + x * (1.0 - s))
+
+
+def relu(x):
+    return np.maximum(x, 0.0)
 
 
 def graph_from_lists(nbr_lists):
@@ -164,54 +158,44 @@ def segsum(vals, idx, n):
     return out
 
 
-def forward(flat, x, v, rho, prs, src, dst, deg, h, rho0, K, consts=None, bf16_round=False,
-            keep=False):
-    """Network output `out` (N x 32; columns 0..2 are the acceleration
-    residual in units of A). With keep=True also returns the intermediates
-    the backward pass uses."""
+def node_vector(S, nu, acc0, deg, g):
+    """u_i = [S_i, nu_i, (a0_i - g) / A, deg_i / C, 0...] (unrounded)."""
+    n = len(nu)
+    u = np.zeros((n, NODE_IN))
+    u[:, :EDGE_W] = S
+    u[:, EDGE_W:EDGE_W + 2] = nu
+    u[:, EDGE_W + 2:EDGE_W + 5] = (np.asarray(acc0, np.float64)[:, :3]
+                                   - np.asarray(g, np.float64)[:3]) / ACC_SCALE
+    u[:, EDGE_W + 5] = np.asarray(deg, np.float64) / AGG_C
+    return u
+
+
+def forward(flat, x, v, rho, prs, acc0, src, dst, deg, h, rho0, K, g, consts=None,
+            bf16_round=False, keep=False):
+    """Network output `out` (N x 32; columns 0..2 are the correction in units
+    of R). acc0 (N x 3 or N x 4) is the forces stage's acceleration and g
+    the constant acceleration. With keep=True also returns the
+    intermediates the backward pass uses."""
     P = unpack(flat)
     R = bf16 if bf16_round else (lambda a: np.asarray(a, np.float64))
     n = len(rho)
-    degc = np.asarray(deg, np.float64)[:, None] / AGG_C
     nu = node_inputs(rho, prs, rho0, K)
     e = R(edge_inputs(x, v, nu, src, dst, h, rho, prs, consts))
-    c = {"e": e, "nu": nu}
-    # round 1 edges
-    z1 = e @ R(P["e1.w1"]) + P["e1.b1"]
-    s1 = R(silu(z1))
-    z2 = s1 @ R(P["e1.w2"]) + P["e1.b2"]
-    S = R(segsum(silu(z2), dst, n) / AGG_C)
-    M1 = S @ R(P["e1.w3"]) + degc * P["e1.b3"]
-    c.update(r1_z1=z1, r1_s1=s1, r1_z2=z2, r1_S=S)
-    # node MLP 1
-    u1 = np.zeros((n, NODE_IN1))
-    u1[:, :WIDTH] = M1
-    u1[:, WIDTH:WIDTH + 2] = nu
-    u1 = R(u1)
-    p1 = u1 @ R(P["n1.w1"]) + P["n1.b1"]
+    z = e @ R(P["e.w1"]) + P["e.b1"]
+    S = R(segsum(relu(z), dst, n) / AGG_C)
+    u = R(node_vector(S, nu, acc0, deg, g))
+    p1 = u @ R(P["n.w1"]) + P["n.b1"]
     y1 = R(silu(p1))
-    p2 = y1 @ R(P["n1.w2"]) + P["n1.b2"]
+    p2 = y1 @ R(P["n.w2"]) + P["n.b2"]
     y2 = R(silu(p2))
-    H = R(y2 @ R(P["n1.w3"]) + P["n1.b3"])
-    c.update(n1_u=u1, n1_p1=p1, n1_y1=y1, n1_p2=p2, n1_y2=y2, H=H)
-    # round 2 edges
-    PA = R(H @ R(P["e2.wa"]))
-    PB = R(H @ R(P["e2.wb"]))
-    z1 = e @ R(P["e2.w1"]) + P["e2.b1"] + PA[dst] + PB[src]
-    s1 = R(silu(z1))
-    z2 = s1 @ R(P["e2.w2"]) + P["e2.b2"]
-    S = R(segsum(silu(z2), dst, n) / AGG_C)
-    M2 = S @ R(P["e2.w3"]) + degc * P["e2.b3"]
-    c.update(r2_z1=z1, r2_s1=s1, r2_z2=z2, r2_S=S)
-    # node MLP 2
-    u2 = R(np.concatenate([H, M2], axis=1))
-    q1 = u2 @ R(P["n2.w1"]) + P["n2.b1"]
-    w1 = R(silu(q1))
-    q2 = w1 @ R(P["n2.w2"]) + P["n2.b2"]
-    w2 = R(silu(q2))
-    out = w2 @ R(P["n2.w3"]) + P["n2.b3"]
-    c.update(n2_u=u2, n2_p1=q1, n2_y1=w1, n2_p2=q2, n2_y2=w2, out=out)
+    out = y2 @ R(P["n.w3"]) + P["n.b3"]
+    c = dict(e=e, z=z, S=S, u=u, p1=p1, y1=y1, p2=p2, y2=y2, out=out)
     return (out, c) if keep else out
+
+
+def acceleration(out, acc0):
+    """a_i = a0_i + R out_i[0:3]."""
+    return np.asarray(acc0, np.float64)[:, :3] + RES_SCALE * np.asarray(out)[:, :3]
 
 
 def loss(out, target):
@@ -225,54 +209,31 @@ def backward(flat, c, target, src, dst, deg):
     parameter vector, from the intermediates of forward(..., keep=True)."""
     P = unpack(flat)
     n = len(target)
-    degc = np.asarray(deg, np.float64)[:, None] / AGG_C
     G = {name: np.zeros(shape) for name, shape in LAYOUT}
     dout = np.zeros((n, OUT))
     dout[:, :3] = 2.0 * (c["out"][:, :3] - target) / n
-
-    def mlp3_back(prefix, u, p1, y1, p2, y2, dy):
-        G[prefix + ".w3"] += y2.T @ dy
-        G[prefix + ".b3"] += dy.sum(0)
-        dp2 = (dy @ P[prefix + ".w3"].T) * dsilu(p2)
-        G[prefix + ".w2"] += y1.T @ dp2
-        G[prefix + ".b2"] += dp2.sum(0)
-        dp1 = (dp2 @ P[prefix + ".w2"].T) * dsilu(p1)
-        G[prefix + ".w1"] += u.T @ dp1
-        G[prefix + ".b1"] += dp1.sum(0)
-        return dp1 @ P[prefix + ".w1"].T
-
-    def edges_back(prefix, r, dM):
-        S = c[r + "_S"]
-        G[prefix + ".w3"] += S.T @ dM
-        G[prefix + ".b3"] += (degc * dM).sum(0)
-        dS = (dM @ P[prefix + ".w3"].T) / AGG_C
-        dz2 = dS[dst] * dsilu(c[r + "_z2"])
-        G[prefix + ".w2"] += c[r + "_s1"].T @ dz2
-        G[prefix + ".b2"] += dz2.sum(0)
-        dz1 = (dz2 @ P[prefix + ".w2"].T) * dsilu(c[r + "_z1"])
-        G[prefix + ".w1"] += c["e"].T @ dz1
-        G[prefix + ".b1"] += dz1.sum(0)
-        return dz1
-
-    du2 = mlp3_back("n2", c["n2_u"], c["n2_p1"], c["n2_y1"], c["n2_p2"], c["n2_y2"], dout)
-    dH = du2[:, :WIDTH].copy()
-    dz1 = edges_back("e2", "r2", du2[:, WIDTH:])
-    dPA = segsum(dz1, dst, n)
-    dPB = segsum(dz1, src, n)
-    G["e2.wa"] += c["H"].T @ dPA
-    G["e2.wb"] += c["H"].T @ dPB
-    dH += dPA @ P["e2.wa"].T + dPB @ P["e2.wb"].T
-    du1 = mlp3_back("n1", c["n1_u"], c["n1_p1"], c["n1_y1"], c["n1_p2"], c["n1_y2"], dH)
-    edges_back("e1", "r1", du1[:, :WIDTH])
+    G["n.w3"] += c["y2"].T @ dout
+    G["n.b3"] += dout.sum(0)
+    dp2 = (dout @ P["n.w3"].T) * dsilu(c["p2"])
+    G["n.w2"] += c["y1"].T @ dp2
+    G["n.b2"] += dp2.sum(0)
+    dp1 = (dp2 @ P["n.w2"].T) * dsilu(c["p1"])
+    G["n.w1"] += c["u"].T @ dp1
+    G["n.b1"] += dp1.sum(0)
+    du = dp1 @ P["n.w1"].T
+    dS = du[:, :EDGE_W] / AGG_C
+    dz = dS[dst] * (c["z"] > 0.0)
+    G["e.w1"] += c["e"].T @ dz
+    G["e.b1"] += dz.sum(0)
     return pack(G)
 
 
 def init_params(seed=0):
     """Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights over the inputs that
-    are used, zero biases, zero rows for the padded inputs and a last layer
-    scaled by 0.1."""
+    are used, zero biases, zero rows for the padded inputs and an output
+    layer scaled by 0.1 with only its first 3 columns nonzero."""
     rng = np.random.default_rng(seed)
-    used_in = {"e1.w1": FEAT_USED, "e2.w1": FEAT_USED, "n1.w1": WIDTH + 2}
+    used_in = {"e.w1": FEAT_USED, "n.w1": NODE_IN_USED}
     P = {}
     for name, shape in LAYOUT:
         if len(shape) == 1:
@@ -281,7 +242,7 @@ def init_params(seed=0):
         fan = used_in.get(name, shape[0])
         w = rng.uniform(-1.0, 1.0, size=shape) / np.sqrt(fan)
         w[fan:] = 0.0
-        if name == "n2.w3":
+        if name == "n.w3":
             w[:, 3:] = 0.0
             w *= 0.1
         P[name] = w
@@ -321,7 +282,7 @@ def random_params(seed, bias_scale=0.1):
     for name, shape in LAYOUT:
         if len(shape) == 1:
             b = rng.normal(0.0, bias_scale, size=shape)
-            if name == "n2.b3":
+            if name == "n.b3":
                 b[3:] = 0.0
             P[name] = b
     return pack(P)

@@ -3,7 +3,7 @@
 On a dam-break variant the training never saw, from one common state:
   (a) the plain method at dt                (the reference trajectory)
   (b) the plain method at 4 dt              (the method without the network)
-  (c) the learned method at 4 dt            (the network in place of forces)
+  (c) the learned method at 4 dt            (the plain substep plus the network's correction)
 Densities of the same particles are compared at the matched time. The
 learned run must stay finite with its density error against (a) inside the
 judge's bound, and the test prints what it gains over (b). A separate case

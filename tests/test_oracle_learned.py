@@ -3,8 +3,9 @@
 The network is this project's own definition, so there is no reference
 output to match. The oracle is pinned by:
   - central finite differences of its FP64 forward (its backward);
-  - a literal per-edge evaluation of round 1 with the third edge layer
-    applied on every edge (the sum-then-project restructuring);
+  - a literal particle-by-particle evaluation with the pair features written
+    out from the forces kernel's formulas (the vectorised gather/scatter);
+  - the zero network returning the forces-stage acceleration exactly;
   - invariance to the order of each particle's neighbours and to a
     translation of all positions;
   - bfloat16 round-to-nearest-even on hand-worked bit patterns;
@@ -32,16 +33,17 @@ def tiny_graph(n=14, seed=0, h=0.01):
     m = 1000.0 * (h / 2) ** 3
     consts = (m, 3.5e-3, -45 / (np.pi * h ** 6), 45 / (np.pi * h ** 6),
               -945 / (32 * np.pi * h ** 9), -945 / (32 * np.pi * h ** 9))
-    return dict(x=x, v=v, rho=rho, prs=prs, src=src, dst=dst, deg=deg, h=h, rho0=1000.0, K=3.0,
-                consts=consts)
+    acc0 = rng.normal(0, 20, (n, 3))
+    return dict(x=x, v=v, rho=rho, prs=prs, acc0=acc0, src=src, dst=dst, deg=deg, h=h,
+                rho0=1000.0, K=3.0, g=np.array([0.0, -9.81, 0.0]), consts=consts)
 
 
 def test_param_layout():
-    assert L.param_count() == 212384
+    assert L.param_count() == 11456
     flat = L.init_params(0)
     P = L.unpack(flat)
-    assert np.all(P["e1.w1"][22:] == 0) and np.all(P["e2.w1"][22:] == 0)
-    assert np.all(P["n1.w1"][130:] == 0) and np.all(P["n2.w3"][:, 3:] == 0)
+    assert np.all(P["e.w1"][22:] == 0) and np.all(P["n.w1"][38:] == 0)
+    assert np.all(P["n.w3"][:, 3:] == 0)
     assert np.array_equal(L.pack(P), flat)
 
 
@@ -75,28 +77,32 @@ def test_padded_parameters_get_zero_gradient():
     target = np.ones((len(g["rho"]), 3))
     out, c = L.forward(flat, keep=True, **g)
     G = L.unpack(L.backward(flat, c, target, g["src"], g["dst"], g["deg"]))
-    assert np.all(G["e1.w1"][22:] == 0) and np.all(G["e2.w1"][22:] == 0)
-    assert np.all(G["n1.w1"][130:] == 0)
-    assert np.all(G["n2.w3"][:, 3:] == 0) and np.all(G["n2.b3"][3:] == 0)
+    assert np.all(G["e.w1"][22:] == 0) and np.all(G["n.w1"][38:] == 0)
+    assert np.all(G["n.w3"][:, 3:] == 0) and np.all(G["n.b3"][3:] == 0)
+    assert np.abs(G["e.w1"][:22]).max() > 0 and np.abs(G["n.w1"][:38]).max() > 0
 
 
-def test_sum_then_project_equals_per_edge_layer():
+def test_literal_per_particle_evaluation():
     g = tiny_graph(seed=6)
     flat = L.random_params(7)
     P = L.unpack(flat)
-    out, c = L.forward(flat, keep=True, **g)
+    out = L.forward(flat, **g)
     n = len(g["rho"])
-    nu = L.node_inputs(g["rho"], g["prs"], g["rho0"], g["K"])
-    M1 = np.zeros((n, L.WIDTH))
+    h = g["h"]
+    m, mu, c_spiky, c_visc, c_grad, c_lap = g["consts"]
+    silu = lambda z: z / (1.0 + np.exp(-z))
     for i in range(n):
+        S = np.zeros(L.EDGE_W)
+        deg = 0
+        nu_i = [g["rho"][i] / g["rho0"] - 1.0, np.arcsinh(g["prs"][i] / g["K"]) / 4.0]
         for j in range(n):
-            if j == i or np.linalg.norm(g["x"][i] - g["x"][j]) >= g["h"]:
+            if j == i or np.linalg.norm(g["x"][i] - g["x"][j]) >= h:
                 continue
+            deg += 1
             d = g["x"][i] - g["x"][j]
             r = np.sqrt(d @ d)
-            q = r / g["h"]
-            h = g["h"]
-            m, mu, c_spiky, c_visc, c_grad, c_lap = g["consts"]
+            q = r / h
+            nu_j = [g["rho"][j] / g["rho0"] - 1.0, np.arcsinh(g["prs"][j] / g["K"]) / 4.0]
             ps = (g["prs"][i] / g["rho"][i] ** 2 + g["prs"][j] / g["rho"][j] ** 2) * m
             pres = -ps * c_spiky * (h - r) ** 2 / r * d / L.ACC_SCALE
             visc = mu / g["rho"][i] * (m / g["rho"][j]) * c_visc * (h - r) * \
@@ -105,11 +111,21 @@ def test_sum_then_project_equals_per_edge_layer():
             grad = (m / g["rho"][j]) * c_grad * t * t * h * d
             lap = (m / g["rho"][j]) * c_lap * t * (3 * h * h - 7 * r * r) * h * h
             e = np.concatenate([d / h, g["v"][i] - g["v"][j], [q, (1 - q * q) ** 3],
-                                nu[i], nu[j], pres, visc, grad, [lap], np.zeros(10)])
-            s1 = L.silu(e @ P["e1.w1"] + P["e1.b1"])
-            s2 = L.silu(s1 @ P["e1.w2"] + P["e1.b2"])
-            M1[i] += (s2 @ P["e1.w3"] + P["e1.b3"]) / L.AGG_C
-    assert np.allclose(c["n1_u"][:, :L.WIDTH], M1, rtol=1e-12, atol=1e-12)
+                                nu_i, nu_j, pres, visc, grad, [lap], np.zeros(10)])
+            S += np.maximum(e @ P["e.w1"] + P["e.b1"], 0.0) / L.AGG_C
+        u = np.concatenate([S, nu_i, (g["acc0"][i] - g["g"]) / L.ACC_SCALE, [deg / L.AGG_C],
+                            np.zeros(26)])
+        y1 = silu(u @ P["n.w1"] + P["n.b1"])
+        y2 = silu(y1 @ P["n.w2"] + P["n.b2"])
+        want = y2 @ P["n.w3"] + P["n.b3"]
+        assert np.allclose(out[i], want, rtol=1e-11, atol=1e-12), i
+
+
+def test_zero_network_returns_the_forces_stage_acceleration():
+    g = tiny_graph(seed=11)
+    out = L.forward(np.zeros(L.param_count()), **g)
+    assert np.all(out == 0.0)
+    assert np.array_equal(L.acceleration(out, g["acc0"]), g["acc0"])
 
 
 def test_invariance_to_neighbour_order_and_translation():
