@@ -113,6 +113,8 @@ def main():
                     help="also track particle identities each substep; its copy kernel "
                          "(k_copy_u32, 4 n bytes read and written) calibrates HBM counters")
     ap.add_argument("--learned-macro-steps", type=int, default=100)
+    ap.add_argument("--judge-every", type=int, default=8,
+                    help="learned macro steps between checks against the plain method")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -186,7 +188,8 @@ def main():
 
     learned = None
     if args.workload == "dam_break" and not args.no_learned:
-        learned = bench_learned(ctx, st, scene, ms_per_step, args.learned_macro_steps)
+        learned = bench_learned(ctx, st, scene, ms_per_step, args.learned_macro_steps,
+                                judge_every=args.judge_every)
 
     cfg.update({
         "dt": float(scene.params["time_delta"]),
@@ -246,11 +249,13 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 def bench_learned(ctx, plain, scene, plain_ms_per_step, macro_steps, judge_every=8, tol=0.01):
     """The learned surrogate (trained weights shipped in-tree) against the
-    plain method from the same state: simulated seconds per wall second of
+    plain method over the same simulated interval from the same state: the
+    plain substeps timed over that interval, then the learned macro steps
+    without and with the judge. Reports simulated seconds per wall second of
     each, the judge's fallbacks, and the density error against the plain run
-    at dt at the matched time (max and mean |rho - rho_plain| / rho0), for the
-    learned run and for the plain method at the same 4 dt step (the method
-    without the network), beside the literal max |rho / rho0 - 1| of both."""
+    at dt at the matched time (max and mean |rho - rho_plain| / rho0), for
+    the learned run and for the plain method at the same 4 dt step (the
+    method without the network), beside the literal max |rho / rho0 - 1|."""
     from libclsph_amd import learned as lrn
     from libclsph_amd import native
     if not os.path.exists(lrn.WEIGHTS):
@@ -261,21 +266,33 @@ def bench_learned(ctx, plain, scene, plain_ms_per_step, macro_steps, judge_every
         return {"skipped": f"{lrn.WEIGHTS} holds {weights.size} parameters, the network "
                            f"{native.lib.clsph_nn_param_count()}"}
     dt = float(scene.params["time_delta"])
+    start = lrn.make_state(ctx, scene)
     learner = lrn.make_state(ctx, scene, dt=lrn.MACRO * dt)
     teacher = lrn.make_state(ctx, scene)
     big = lrn.make_state(ctx, scene, dt=lrn.MACRO * dt)
     net = native.Network(ctx, n, e_cap=1024)
     net.set_params(weights)
     plain.track_ids(True)
-    for s in (learner, teacher, big):
+    for s in (start, learner, teacher, big):
         s.track_ids(True)
-    learner.copy_from(plain)
-    big.copy_from(plain)
-    # kernel timing of the network on the current state
+    start.copy_from(plain)
+    # the network's own time on the start state
+    learner.copy_from(start)
     lrn.forces_stages(learner)
     net_ms, edges = net.profile(learner, reps=5)
-    r = lrn.run_learned(ctx, learner, teacher, net, macro_steps, dt, judge_every, tol)
+    # plain substeps over the interval
+    ctx.sync()
+    t0 = time.perf_counter()
     plain.step(macro_steps * lrn.MACRO)
+    ctx.sync()
+    plain_wall = time.perf_counter() - t0
+    # learned macro steps over the same interval, without the judge
+    learner.copy_from(start)
+    bare = lrn.run_learned(ctx, learner, teacher, net, macro_steps, dt, judge_every=0)
+    # and with it
+    learner.copy_from(start)
+    r = lrn.run_learned(ctx, learner, teacher, net, macro_steps, dt, judge_every, tol)
+    big.copy_from(start)
     big.step(macro_steps)
     for s in (plain, learner, big):
         lrn.density_stages(s)
@@ -286,34 +303,31 @@ def bench_learned(ctx, plain, scene, plain_ms_per_step, macro_steps, judge_every
     for name, s in (("learned", learner), ("plain", plain)):
         d = s.download_sorted()["density"].astype(np.float64)
         lit[name] = float(np.abs(d / rho0 - 1.0).max())
-    # learned macro steps without the judge, for the network's own cost
-    learner.copy_from(plain)
-    ctx.sync()
-    t0 = time.perf_counter()
-    learner.learned_step(net, 20)
-    ctx.sync()
-    bare_ms = (time.perf_counter() - t0) / 20 * 1e3
-    plain_rate = dt / (plain_ms_per_step * 1e-3)
-    learned_rate = r.sim_s / r.wall_s
-    for s in (learner, teacher, big):
+    for s in (start, learner, teacher, big):
         s.close()
     plain.track_ids(False)
     net.close()
+    sim_s = macro_steps * lrn.MACRO * dt
+    plain_rate = sim_s / plain_wall
     tflops = (edges * NN_EDGE_FLOP + n * NN_NODE_FLOP) / (net_ms * 1e-3) / 1e12
     return {
-        "sim_s_per_wall_s": {"plain": plain_rate, "learned": learned_rate,
-                             "ratio": learned_rate / plain_rate,
-                             "ratio_unjudged": lrn.MACRO * plain_ms_per_step / bare_ms},
+        "interval": {"substeps": macro_steps * lrn.MACRO, "sim_s": sim_s,
+                     "plain_ms_per_substep": plain_wall / (macro_steps * lrn.MACRO) * 1e3},
+        "sim_s_per_wall_s": {"plain": plain_rate, "learned": r.sim_s / r.wall_s,
+                             "ratio": (r.sim_s / r.wall_s) / plain_rate,
+                             "learned_unjudged": bare.sim_s / bare.wall_s,
+                             "ratio_unjudged": (bare.sim_s / bare.wall_s) / plain_rate},
         "macro_step_dt": lrn.MACRO * dt,
         "macro_steps": r.macro_steps,
+        "ms_per_macro_step": {"unjudged": bare.wall_s / macro_steps * 1e3,
+                              "judged": r.wall_s / macro_steps * 1e3},
         "judge_every": judge_every,
         "judged": r.judged,
         "fallbacks": r.fallbacks,
         "judge_max_density_err": max(r.judge_err) if r.judge_err else None,
-        "density_err_vs_plain": {"max": err_max, "mean": err_mean, "sim_time_s": r.sim_s},
+        "density_err_vs_plain": {"max": err_max, "mean": err_mean},
         "plain_4dt_density_err_vs_plain": {"max": big_max, "mean": big_mean},
         "literal_max_abs_rho_over_rho0_minus_1": lit,
-        "ms_per_macro_step_unjudged": bare_ms,
         "network_ms": net_ms,
         "edges": edges,
         "network_kernel_mfma": {

@@ -238,6 +238,17 @@ int clsph_nn_adam(clsph_nn* nn, float lr, float beta1, float beta2, float eps);
 /* `nsteps` substeps at the state's time_delta, each the plain substep with
  * the network's correction added to the forces stage's accelerations. */
 int clsph_learned_step(clsph_state* st, clsph_nn* nn, uint32_t nsteps);
+/* `macro_steps` learned steps of `learner` (time_delta = the macro step,
+ * 4 plain substeps), judged every `judge_every` steps (0: never) by `judge`
+ * (time_delta = the plain substep, identities tracked in both): the judge
+ * starts from the learner's particles and runs 4 plain substeps, both get
+ * their densities, and when max |rho - rho_judge| / fluid_density exceeds
+ * `tol` the learner takes the judge's particles (a fallback). Writes the
+ * counts and each judged step's error (judge_err: one float per judged step,
+ * optional). One host synchronisation per judged step. */
+int clsph_learned_run(clsph_state* learner, clsph_state* judge, clsph_nn* nn,
+                      uint32_t macro_steps, uint32_t judge_every, float tol, uint32_t* judged,
+                      uint32_t* fallbacks, float* judge_err);
 /* Density difference of the same particles in two states after both ran
  * their density stage (identities tracked in both): max and mean of
  * |rho_a - rho_b| / fluid_density. */

@@ -42,6 +42,8 @@ void free_state_buffers(clsph_state* st) {
   for (auto& b : st->blocks) dfree(b);
   if (st->graph) (void)hipGraphExecDestroy(st->graph);
   st->graph = nullptr;
+  if (st->lgraph) (void)hipGraphExecDestroy(st->lgraph);
+  st->lgraph = nullptr;
 }
 
 // Work-group tables for every block size, and each scene's range in the
@@ -63,11 +65,12 @@ int build_blocks(clsph_state* st) {
                                hipMemcpyHostToDevice));
     st->nblocks[b] = (uint32_t)t.size();
   }
-  st->graph_dirty = true;
+  st->graph_dirty = st->lgraph_dirty = true;
   return CLSPH_OK;
 }
 
 int upload_params(clsph_state* st) {
+  st->lgraph_dirty = true;  // the learned step's graph holds parameters by value
   for (uint32_t s = 0; s < st->n_scenes; ++s) {
     st->host_dps[s].begin = st->scene_begin[s];
     st->host_dps[s].n = st->scene_n[s];
@@ -212,7 +215,7 @@ int rebuild_meshes(clsph_state* st) {
   if (!rc) rc = up(st->tgrid_start, starts.data(), 4 * starts.size());
   if (!rc) rc = up(st->tgrid_tris, tris.data(), 4 * tris.size());
   if (rc) return rc;
-  st->graph_dirty = true;
+  st->graph_dirty = st->lgraph_dirty = true;
   // Only the mesh fields change; the rest of the device parameters is kept.
   for (uint32_t s = 0; s < st->n_scenes; ++s) {
     const DevParams& h = st->host_dps[s];
@@ -707,7 +710,7 @@ int clsph_set_block_sizes(clsph_state* st, const uint32_t sizes[7]) {
   HIPCK(st->ctx, hipStreamSynchronize(st->ctx->stream));
   bool advect_changed = idx[kStageAdvect] != st->stage_bs[kStageAdvect];
   for (int s = 0; s < kNumStages; ++s) st->stage_bs[s] = idx[s];
-  st->graph_dirty = true;
+  st->graph_dirty = st->lgraph_dirty = true;
   if (advect_changed) {
     // the bounding-box partials are laid out per advection work-group:
     // rebuild the scene ranges and recompute them from the current state

@@ -35,6 +35,7 @@ constexpr float kVelScale = 1.f;
 constexpr float kAccScale = 10.f;  // A
 constexpr float kResScale = 2.f;   // R
 constexpr uint32_t kMaxEdges = 128;  // neighbours per particle the network sees
+constexpr uint32_t kMacro = 4;       // substeps per learned macro step
 
 // Flat parameter vector: row-major (inputs x outputs) matrices and biases.
 enum Param { E_W1, E_B1, N_W1, N_B1, N_W2, N_B2, N_W3, N_B3, kNumParams };
@@ -53,6 +54,7 @@ constexpr uint32_t kParamTotal = param_offset(kNumParams);
 
 struct clsph_nn {
   clsph_ctx* ctx = nullptr;
+  uint64_t serial = 0;  // unique per workspace (graphs captured for it name it)
   uint32_t n_cap = 0;  // particles
   uint32_t e_cap = 0;  // edges whose activations the training buffers hold
 
@@ -108,12 +110,15 @@ void launch_edge_inputs(clsph_nn* m, clsph_state* s);
 // `acc0` and leaves the output in m->out; the state is not changed.
 void forward(clsph_nn* m, const float4* acc0, float gx, float gy, float gz);
 // Inference forward, one fused kernel over the state's neighbour lists (no
-// per-edge storage, no edge-count readback): acc[i] += R out_i[0:3] where
-// acc holds the forces stage's acceleration. With `out` non-null the full
-// output is written there as well. With `ev` (2 events) the stream gets
-// events before and after the kernel.
-void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float* out, float gx, float gy,
-                   float gz, hipEvent_t* ev = nullptr);
+// per-edge storage, no edge-count readback): acc[i] = a0_i + R out_i[0:3].
+// a0 is acc[i] as the forces stage left it, or with `own_forces` the forces
+// stage's acceleration summed in the same kernel from the listed neighbours
+// (then acc must hold the forces stage's result for the particles whose list
+// overflowed: launch_forces_overflow). With `out` non-null the full output
+// is written there as well. With `ev` (2 events) the stream gets events
+// before and after the kernel.
+void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float* out, bool own_forces,
+                   hipEvent_t* ev = nullptr);
 // Loss against m->target and gradients into m->grad (zeroed first).
 void backward(clsph_nn* m);
 void adam(clsph_nn* m, float lr, float beta1, float beta2, float eps);

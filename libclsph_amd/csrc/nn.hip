@@ -275,6 +275,60 @@ inline EdgeConsts edge_consts(const DevParams& d) {
   return k;
 }
 
+// Constants of the forces stage (forces.cl:45-112, sph.cl:53-58) for the
+// fused inference, which sums the listed neighbours' pair terms itself.
+struct ForceTail {
+  float h2, spiky0, mu, sigma, thresh, gx, gy, gz;
+};
+
+inline ForceTail force_tail(const DevParams& d) {
+  ForceTail t;
+  t.h2 = d.p.h * d.p.h;
+  const float h3 = t.h2 * d.p.h;
+  t.spiky0 = -45.f / (3.14159274f * (h3 * h3));  // coincident particles (smoothing.cl:23-25)
+  t.mu = d.p.dynamic_viscosity;
+  t.sigma = d.p.surface_tension;
+  t.thresh = d.p.surface_tension_threshold;
+  t.gx = d.p.constant_acceleration.s[0];
+  t.gy = d.p.constant_acceleration.s[1];
+  t.gz = d.p.constant_acceleration.s[2];
+  return t;
+}
+
+// Pair sums of the forces stage for particle i: pressure (without the
+// -rho_i factor), viscosity (without mu), colour-field gradient and
+// laplacian, as the forces kernel accumulates them (forces.cl:60-101).
+struct PairSums {
+  float px, py, pz, vx, vy, vz, nx, ny, nz, lap;
+};
+
+__device__ __attribute__((always_inline)) inline void pair_terms(PairSums& f, const EdgeConsts& k,
+                                                                 const ForceTail& ft, float4 ai,
+                                                                 float4 bi, float4 aj, float4 bj,
+                                                                 float w) {
+  float dx = ai.x - aj.x, dy = ai.y - aj.y, dz = ai.z - aj.z;
+  float d2 = dx * dx + dy * dy + dz * dz;
+  float r = __builtin_amdgcn_sqrtf(d2);
+  float t = ft.h2 - d2;
+  float hr = k.h - r;
+  float ps = (aj.w + ai.w) * k.m;
+  const bool coincident = r < 1e-7f;
+  float g = coincident ? ps * ft.spiky0 : ps * k.c_spiky * hr * hr * __builtin_amdgcn_rcpf(r);
+  g *= w;
+  f.px += g * (coincident ? 1.f : dx);
+  f.py += g * (coincident ? 1.f : dy);
+  f.pz += g * (coincident ? 1.f : dz);
+  float vw = bj.w * k.c_visc * hr * w;
+  f.vx += (bj.x - bi.x) * vw;
+  f.vy += (bj.y - bi.y) * vw;
+  f.vz += (bj.z - bi.z) * vw;
+  float gw = bj.w * k.c_grad * t * t * w;
+  f.nx += gw * dx;
+  f.ny += gw * dy;
+  f.nz += gw * dz;
+  f.lap += bj.w * k.c_lap * t * (3.f * ft.h2 - 7.f * d2) * w;
+}
+
 // The 32 edge inputs of edge (i <- j) in float32 (22 used, 10 zero), from
 // the density stage's per-particle force inputs a = (x, p / rho^2) and
 // b = (v, m / rho) and the node inputs nu. The pair terms are the ones the
@@ -432,29 +486,48 @@ __device__ __attribute__((always_inline)) inline void silu_operands(const f32x4*
   }
 }
 
-// One round of 4 edge steps for the wave's tile (see above). MASKED rounds
-// contain edges past some column's degree: those edges store zero inputs and
-// their steps start from a zero accumulator, so they add relu(0) = 0.
-template <bool MASKED>
-__device__ __attribute__((always_inline)) inline void edge_round(
-    uint32_t k, uint32_t i, uint32_t n, uint32_t deg, const float4& ai, const float4& bi,
-    const float2& ni, const float4* __restrict__ fa, const float4* __restrict__ fb,
-    const float2* __restrict__ nu, const uint4* __restrict__ nbr, const EdgeConsts& kc,
-    uint8_t* wlds, const u32x4* __restrict__ w1s, const f32x4* zb, f32x4* s) {
-  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
-  // this lane's edge: 4 k + g of particle c
-  const bool mine = !MASKED || 4 * k + g < deg;
+// The neighbour of lane (g, c) in round k: edge 4 k + g of particle c (the
+// particle itself when that edge is past its degree), with its inputs.
+struct EdgeFetch {
+  float4 a, b;
+  float2 nu;
+  bool mine;
+};
+
+__device__ __attribute__((always_inline)) inline EdgeFetch fetch_edge(
+    uint32_t k, uint32_t i, uint32_t n, uint32_t deg, const float4* __restrict__ fa,
+    const float4* __restrict__ fb, const float2* __restrict__ nu,
+    const uint4* __restrict__ nbr) {
+  const int g = (threadIdx.x & 63) >> 4;
+  EdgeFetch e;
+  e.mine = 4 * k + g < deg;
   uint32_t j = i;
-  if (mine) {
+  if (e.mine) {
     uint4 js = nbr[(size_t)k * n + i];
     j = g == 0 ? js.x : g == 1 ? js.y : g == 2 ? js.z : js.w;
   }
+  e.a = fa[j];
+  e.b = fb[j];
+  e.nu = nu[j];
+  return e;
+}
+
+// One round of 4 edge steps for the wave's tile (see above). MASKED rounds
+// contain edges past some column's degree: those edges store zero inputs and
+// their steps start from a zero accumulator, so they add relu(0) = 0.
+template <bool MASKED, bool OWN>
+__device__ __attribute__((always_inline)) inline void edge_round(
+    uint32_t k, uint32_t deg, const float4& ai, const float4& bi, const float2& ni,
+    const EdgeFetch& ej, const EdgeConsts& kc, const ForceTail& ft, uint8_t* wlds,
+    const u32x4* __restrict__ w1s, const f32x4* zb, f32x4* s, PairSums& ps) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  if (OWN) pair_terms(ps, kc, ft, ai, bi, ej.a, ej.b, (MASKED && !ej.mine) ? 0.f : 1.f);
   float f[kFeat];
-  edge_features(ai, bi, fa[j], fb[j], ni, nu[j], kc, f);
+  edge_features(ai, bi, ej.a, ej.b, ni, ej.nu, kc, f);
   uint8_t* col = wlds + c * kColBytes;
   for (int q = 0; q < 3; ++q) {
     u32x4 v = pack8(f + 8 * q);
-    if (MASKED && !mine) v = u32x4{0u, 0u, 0u, 0u};
+    if (MASKED && !ej.mine) v = u32x4{0u, 0u, 0u, 0u};
     *(u32x4*)(col + g * 48 + 16 * q) = v;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -480,11 +553,17 @@ __device__ __attribute__((always_inline)) inline void edge_round(
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// OWN: the forces stage's acceleration a0 is summed here from the listed
+// neighbours (particles whose list overflowed take acc[i], which the
+// forces kernel wrote for them); otherwise a0 = acc[i].
+template <bool OWN>
 __global__ __launch_bounds__(kSurWaves * 64) void k_surrogate(
     const float4* __restrict__ fa, const float4* __restrict__ fb, const float2* __restrict__ nu,
-    const uint4* __restrict__ nbr, const uint32_t* __restrict__ count, uint32_t n, EdgeConsts kc,
+    const float* __restrict__ rho, const uint4* __restrict__ nbr,
+    const uint32_t* __restrict__ count, uint32_t n, EdgeConsts kc, ForceTail ft,
     const uint16_t* __restrict__ wb, const float* __restrict__ params, float4* __restrict__ acc,
-    float* __restrict__ out, float gx, float gy, float gz) {
+    float* __restrict__ out) {
+  const float gx = ft.gx, gy = ft.gy, gz = ft.gz;
   __shared__ u32x4 w1s[2 * 64];          // edge layer: 2 output blocks, 1 k-step
   __shared__ u32x4 wn1[4 * 2 * 64];      // node layer 1: k-step 0 permuted, 1 natural
   __shared__ u32x4 wn2[4 * 2 * 64];
@@ -523,18 +602,58 @@ __global__ __launch_bounds__(kSurWaves * 64) void k_surrogate(
       dmin = min(dmin, (uint32_t)__shfl_xor((int)dmin, o, 64));
       dmax = max(dmax, (uint32_t)__shfl_xor((int)dmax, o, 64));
     }
+    const uint32_t cnt = valid ? count[i] : 0u;
     const float4 ai = fa[i], bi = fb[i];
     const float2 ni = nu[i];
     f32x4 s[2];
     for (int ob = 0; ob < 2; ++ob) s[ob] = f32x4{0.f, 0.f, 0.f, 0.f};
+    PairSums ps = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    // rounds inside every column's degree, then the masked tail; each
+    // round's neighbour inputs are fetched one round ahead
     const uint32_t kfull = dmin >> 2, kend = (dmax + 3) >> 2;
-    for (uint32_t k = 0; k < kfull; ++k)
-      edge_round<false>(k, i, n, deg, ai, bi, ni, fa, fb, nu, nbr, kc, wlds, w1s, zb, s);
-    for (uint32_t k = kfull; k < kend; ++k)
-      edge_round<true>(k, i, n, deg, ai, bi, ni, fa, fb, nu, nbr, kc, wlds, w1s, zb, s);
+    EdgeFetch cur;
+    if (kend) cur = fetch_edge(0, i, n, deg, fa, fb, nu, nbr);
+    for (uint32_t k = 0; k < kend; ++k) {
+      EdgeFetch nxt = cur;
+      if (k + 1 < kend) nxt = fetch_edge(k + 1, i, n, deg, fa, fb, nu, nbr);
+      if (k < kfull)
+        edge_round<false, OWN>(k, deg, ai, bi, ni, cur, kc, ft, wlds, w1s, zb, s, ps);
+      else
+        edge_round<true, OWN>(k, deg, ai, bi, ni, cur, kc, ft, wlds, w1s, zb, s, ps);
+      cur = nxt;
+    }
+    float4 a0;
+    if (OWN) {
+      // the four lane groups hold interleaved edges of the same particle
+      auto group_sum = [](float& v) {
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+      };
+      group_sum(ps.px), group_sum(ps.py), group_sum(ps.pz);
+      group_sum(ps.vx), group_sum(ps.vy), group_sum(ps.vz);
+      group_sum(ps.nx), group_sum(ps.ny), group_sum(ps.nz);
+      group_sum(ps.lap);
+      // the forces kernel's closing formula (forces.cl:95-111, sph.cl:53-58),
+      // with the colour-field laplacian's self term
+      const float rho_i = rho[i];
+      const float lap = bi.w * kc.c_lap * ft.h2 * (3.f * ft.h2) + ps.lap;
+      float fx = -rho_i * ps.px + ps.vx * ft.mu;
+      float fy = -rho_i * ps.py + ps.vy * ft.mu;
+      float fz = -rho_i * ps.pz + ps.vz * ft.mu;
+      const float nl = sqrtf(ps.nx * ps.nx + ps.ny * ps.ny + ps.nz * ps.nz);
+      if (nl > ft.thresh) {
+        const float kk = -ft.sigma * lap / nl;
+        fx += kk * ps.nx;
+        fy += kk * ps.ny;
+        fz += kk * ps.nz;
+      }
+      a0 = make_float4(fx / rho_i + gx, fy / rho_i + gy, fz / rho_i + gz, 0.f);
+      if (cnt > kMaxEdges) a0 = acc[i];
+    } else {
+      a0 = acc[i];
+    }
 
     // node layer 1: [S, nu, (a0 - g) / A, deg / C, 0...]
-    const float4 a0 = acc ? acc[i] : make_float4(gx, gy, gz, 0.f);
     bf16x8 bop[2];
     {
       float v8[8];
@@ -562,7 +681,7 @@ __global__ __launch_bounds__(kSurWaves * 64) void k_surrogate(
         for (int ob = 0; ob < 2; ++ob)
           *(float4*)(orow + 16 * ob + 4 * g) = make_float4(o[ob][0], o[ob][1], o[ob][2], o[ob][3]);
       }
-      if (acc && g == 0)
+      if (g == 0)
         acc[i] = make_float4(a0.x + kResScale * o[0][0], a0.y + kResScale * o[0][1],
                              a0.z + kResScale * o[0][2], a0.w);
     }
@@ -751,32 +870,64 @@ __global__ void k_invert_perm(const uint32_t* __restrict__ perm, uint32_t n,
 // Density difference of particle (sorted position i of state a) against the
 // same particle in state b: a's canonical index perm_a[i] carries identity
 // ids_a[...]; b holds it at canonical loc_b[id], sorted inv_b[loc_b[id]].
+// Grid-stride; each work-group writes its (max, sum) to part[2 blockIdx.x].
+constexpr int kCmpBlocks = 1024;
+
 __global__ __launch_bounds__(256) void k_compare(const uint32_t* __restrict__ perm_a,
                                                  const uint32_t* __restrict__ ids_a,
                                                  const float* __restrict__ rho_a,
                                                  const uint32_t* __restrict__ loc_b,
                                                  const uint32_t* __restrict__ inv_b,
                                                  const float* __restrict__ rho_b, uint32_t n,
-                                                 float* __restrict__ res) {
-  __shared__ float red[4];
-  uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  float d = 0.f;
-  if (i < n) {
+                                                 float* __restrict__ part) {
+  __shared__ float red[2][4];
+  float mx = 0.f, sm = 0.f;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     uint32_t c = perm_a[i];
     uint32_t id = c < n ? ids_a[c] : 0u;
     uint32_t cb = id < n ? loc_b[id] : 0u;
     uint32_t ib = cb < n ? inv_b[cb] : 0u;
-    d = fabsf(rho_a[i] - rho_b[ib]);
+    float d = fabsf(rho_a[i] - rho_b[ib]);
+    mx = fmaxf(mx, d);
+    sm += d;
   }
-  float mx = d, sm = d;
   for (int o = 32; o > 0; o >>= 1) {
     mx = fmaxf(mx, __shfl_xor(mx, o, 64));
     sm += __shfl_xor(sm, o, 64);
   }
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sm;
-  if ((threadIdx.x & 63) == 0) atomicMax((uint32_t*)res, __float_as_uint(mx));
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = mx;
+    red[1][threadIdx.x >> 6] = sm;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(res + 1, red[0] + red[1] + red[2] + red[3]);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = fmaxf(fmaxf(red[0][0], red[0][1]), fmaxf(red[0][2], red[0][3]));
+    part[2 * blockIdx.x + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+  }
+}
+
+// res = (max, sum) over the `nparts` work-group partials; one work-group.
+__global__ __launch_bounds__(256) void k_compare_final(const float* __restrict__ part,
+                                                       uint32_t nparts, float* __restrict__ res) {
+  __shared__ float red[2][4];
+  float mx = 0.f, sm = 0.f;
+  for (uint32_t k = threadIdx.x; k < nparts; k += 256) {
+    mx = fmaxf(mx, part[2 * k]);
+    sm += part[2 * k + 1];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    sm += __shfl_xor(sm, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = mx;
+    red[1][threadIdx.x >> 6] = sm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    res[0] = fmaxf(fmaxf(red[0][0], red[0][1]), fmaxf(red[0][2], red[0][3]));
+    res[1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+  }
 }
 
 // Adds the forces stage's acceleration of sorted particle j to its identity's
@@ -870,8 +1021,8 @@ void forward(clsph_nn* m, const float4* acc0, float gx, float gy, float gz) {
        1.f, P(m, N_B3));
 }
 
-void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float* out, float gx, float gy,
-                   float gz, hipEvent_t* ev) {
+void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float* out, bool own_forces,
+                   hipEvent_t* ev) {
   hipStream_t s = m->ctx->stream;
   uint32_t n = st->n;
   m->n = n;
@@ -884,8 +1035,8 @@ void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float* out, float 
   // once each
   static int per_cu = 0;
   if (per_cu <= 0) {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_surrogate, kSurWaves * 64, 0) !=
-            hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_surrogate<true>,
+                                                     kSurWaves * 64, 0) != hipSuccess ||
         per_cu <= 0)
       per_cu = 1;
   }
@@ -893,9 +1044,14 @@ void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float* out, float 
   uint32_t blocks = (uint32_t)st->ctx->cus * (uint32_t)per_cu;
   if (blocks * kSurWaves > ntiles) blocks = (ntiles + kSurWaves - 1) / kSurWaves;
   if (blocks == 0) blocks = 1;
-  hipLaunchKernelGGL(k_surrogate, dim3(blocks), dim3(kSurWaves * 64), 0, s, st->fa, st->fb,
-                     (const float2*)m->nu, st->nbr, st->nbr_count, n, edge_consts(d), m->wb,
-                     m->params, acc, out, gx, gy, gz);
+  if (own_forces)
+    hipLaunchKernelGGL(k_surrogate<true>, dim3(blocks), dim3(kSurWaves * 64), 0, s, st->fa,
+                       st->fb, (const float2*)m->nu, st->rho_s, st->nbr, st->nbr_count, n,
+                       edge_consts(d), force_tail(d), m->wb, m->params, acc, out);
+  else
+    hipLaunchKernelGGL(k_surrogate<false>, dim3(blocks), dim3(kSurWaves * 64), 0, s, st->fa,
+                       st->fb, (const float2*)m->nu, st->rho_s, st->nbr, st->nbr_count, n,
+                       edge_consts(d), force_tail(d), m->wb, m->params, acc, out);
   if (ev) (void)hipEventRecord(ev[1], s);
 }
 
@@ -996,11 +1152,15 @@ void gemm_public(hipStream_t s, bool ta, bool tb, bool add, const uint16_t* A, i
 void launch_compare(clsph_nn* m, clsph_state* a, clsph_state* b, float res[2]) {
   hipStream_t s = m->ctx->stream;
   uint32_t n = a->n;
-  fill(m->cmp, 2, 0.f, s);
   hipLaunchKernelGGL(k_invert, grid_for(n), dim3(256), 0, s, b->ids, n, m->loc);
   hipLaunchKernelGGL(k_invert_perm, grid_for(n), dim3(256), 0, s, b->val_b, n, m->inv);
-  hipLaunchKernelGGL(k_compare, grid_for(n), dim3(256), 0, s, a->val_b, a->ids, a->rho_s, m->loc,
-                     m->inv, b->rho_s, n, m->cmp);
+  uint32_t blocks = (n + 255) / 256;
+  if (blocks > (uint32_t)kCmpBlocks) blocks = kCmpBlocks;
+  if (blocks == 0) blocks = 1;
+  // the partials go to m->sum: 2 ceil(n / 256) <= 2 n <= 3 n_cap floats
+  hipLaunchKernelGGL(k_compare, dim3(blocks), dim3(256), 0, s, a->val_b, a->ids, a->rho_s, m->loc,
+                     m->inv, b->rho_s, n, m->sum);
+  hipLaunchKernelGGL(k_compare_final, dim3(1), dim3(256), 0, s, m->sum, blocks, m->cmp);
   (void)hipMemcpyAsync(res, m->cmp, 2 * sizeof(float), hipMemcpyDeviceToHost, s);
 }
 

@@ -2,6 +2,8 @@
 // and judging use (include/clsph_hip.h, "learned surrogate" section).
 #include <string.h>
 
+#include <atomic>
+
 #include <string>
 #include <vector>
 
@@ -59,6 +61,92 @@ int copy_d(clsph_ctx* ctx, void* dst, const void* src, size_t bytes) {
   return CLSPH_OK;
 }
 
+// Copies the particles and per-particle fields of single-scene `src` into
+// `dst` on the context stream (identities when both track them).
+int copy_state_async(clsph_state* dst, clsph_state* src) {
+  if (!dst || !src) return fail(nullptr, CLSPH_EINVAL, "NULL state");
+  if (dst->n_scenes != 1 || src->n_scenes != 1 || dst->n_cap < src->n)
+    return fail(dst->ctx, CLSPH_EINVAL, "state copy needs single-scene states of enough capacity");
+  clsph_ctx* c = dst->ctx;
+  size_t n = src->n;
+  int rc = copy_d(c, dst->pos, src->pos, sizeof(float4) * n);
+  if (!rc) rc = copy_d(c, dst->vel, src->vel, sizeof(float4) * n);
+  if (!rc) rc = copy_d(c, dst->vh, src->vh, sizeof(float4) * n);
+  if (!rc) rc = copy_d(c, dst->acc_s, src->acc_s, sizeof(float4) * n);
+  if (!rc) rc = copy_d(c, dst->rho_s, src->rho_s, sizeof(float) * n);
+  if (!rc) rc = copy_d(c, dst->prs_s, src->prs_s, sizeof(float) * n);
+  if (!rc) rc = copy_d(c, dst->key_b, src->key_b, sizeof(uint32_t) * n);
+  if (!rc) rc = copy_d(c, dst->slot, src->slot, sizeof(uint32_t) * n);
+  if (!rc && dst->ids && src->ids) rc = copy_d(c, dst->ids, src->ids, sizeof(uint32_t) * n);
+  if (rc) return rc;
+  if (dst->n != src->n) {
+    rc = resize_single(dst, src->n);
+    if (rc) return rc;
+  }
+  launch_aabb(dst);
+  dst->acc_zero = src->acc_zero;
+  dst->acc_learned = false;
+  dst->fields_canonical = src->fields_canonical;
+  dst->stage = -1;
+  return check_launch(c);
+}
+
+// The stages of a substep up to density, as clsph_grid .. clsph_density_pressure.
+void enqueue_density_stages(clsph_state* st) {
+  launch_grid_params(st);
+  launch_hash(st);
+  launch_sort(st);
+  launch_permute(st);
+  launch_cell_table(st);
+  launch_density(st);
+  st->acc_learned = false;
+  st->fields_canonical = false;
+  st->stage = kStageDensity;
+}
+
+// The learned macro step (the plain substep's stages, the network's
+// correction after forces, advection) captured as one HIP graph per state;
+// recaptured when the network, the particle count, the parameters or the
+// work-group tables change.
+int ensure_learned_graph(clsph_state* st, clsph_nn* m) {
+  if (st->lgraph && !st->lgraph_dirty && st->lgraph_serial == m->serial && st->lgraph_n == st->n)
+    return CLSPH_OK;
+  if (st->lgraph) {
+    (void)hipGraphExecDestroy(st->lgraph);
+    st->lgraph = nullptr;
+  }
+  hipStream_t s = st->ctx->stream;
+  hipGraph_t graph = nullptr;
+  HIPCK(st->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  launch_grid_params(st);
+  launch_hash(st);
+  launch_sort(st);
+  launch_permute(st);
+  launch_cell_table(st);
+  launch_density(st);
+  launch_forces_overflow(st);
+  nn::forward_fused(m, st, st->acc_s, nullptr, true);
+  launch_advect(st);
+  if (st->ids) launch_track_ids(st);
+  hipError_t le = hipGetLastError();
+  HIPCK(st->ctx, hipStreamEndCapture(s, &graph));
+  if (le != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return fail(st->ctx, CLSPH_EDEVICE, std::string("capture: ") + hipGetErrorString(le));
+  }
+  hipError_t ie = hipGraphInstantiate(&st->lgraph, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess) {
+    st->lgraph = nullptr;
+    return fail(st->ctx, CLSPH_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(ie));
+  }
+  st->lgraph_dirty = false;
+  st->lgraph_serial = m->serial;
+  st->lgraph_n = st->n;
+  return CLSPH_OK;
+}
+
+
 }  // namespace
 
 extern "C" {
@@ -70,7 +158,9 @@ int clsph_nn_create(clsph_ctx* ctx, uint32_t n_cap, uint32_t e_cap, clsph_nn** o
   *out = nullptr;
   HIPCK(ctx, hipSetDevice(ctx->device));
   clsph_nn* m = new clsph_nn();
+  static std::atomic<uint64_t> next_serial{1};
   m->ctx = ctx;
+  m->serial = next_serial++;
   m->n_cap = n_cap;
   m->e_cap = e_cap ? e_cap : 64u * n_cap;
   size_t N = n_cap, E = m->e_cap, P = nn::kParamTotal;
@@ -164,9 +254,7 @@ int clsph_nn_infer(clsph_nn* m, clsph_state* st) {
   if (st->stage != kStageForces || st->acc_learned)
     return fail(m->ctx, CLSPH_EINVAL,
                 "clsph_nn_infer needs the forces stage of the substep, once");
-  float g[3];
-  gravity(st, g);
-  nn::forward_fused(m, st, st->acc_s, m->out, g[0], g[1], g[2]);
+  nn::forward_fused(m, st, st->acc_s, m->out, false);
   rc = check_launch(m->ctx);
   if (rc) return rc;
   st->acc_learned = true;
@@ -180,8 +268,6 @@ int clsph_nn_profile(clsph_nn* m, clsph_state* st, uint32_t reps, float* ms,
   if (!ms || reps == 0) return fail(m->ctx, CLSPH_EINVAL, "clsph_nn_profile: bad argument");
   if (st->stage != kStageForces)
     return fail(m->ctx, CLSPH_EINVAL, "clsph_nn_profile needs the forces stage of the substep");
-  float g[3];
-  gravity(st, g);
   // the accelerations are copied into dY (n x 64 floats holds n float4s)
   float4* acc = reinterpret_cast<float4*>(m->dY);
   HIPCK(m->ctx, hipMemcpyAsync(acc, st->acc_s, sizeof(float4) * st->n, hipMemcpyDeviceToDevice,
@@ -190,7 +276,7 @@ int clsph_nn_profile(clsph_nn* m, clsph_state* st, uint32_t reps, float* ms,
   for (auto& e : ev) HIPCK(m->ctx, hipEventCreate(&e));
   double total = 0.0;
   for (uint32_t r = 0; r < reps && !rc; ++r) {
-    nn::forward_fused(m, st, acc, nullptr, g[0], g[1], g[2], ev);
+    nn::forward_fused(m, st, acc, nullptr, true, ev);
     rc = check_launch(m->ctx);
     if (rc) break;
     if (hipEventSynchronize(ev[1]) != hipSuccess) {
@@ -313,32 +399,9 @@ int clsph_nn_adam(clsph_nn* m, float lr, float beta1, float beta2, float eps) {
 }
 
 int clsph_state_copy(clsph_state* dst, clsph_state* src) {
-  if (!dst || !src) return fail(nullptr, CLSPH_EINVAL, "NULL state");
-  if (dst->n_scenes != 1 || src->n_scenes != 1 || dst->n_cap < src->n)
-    return fail(dst->ctx, CLSPH_EINVAL, "state copy needs single-scene states of enough capacity");
-  clsph_ctx* c = dst->ctx;
-  size_t n = src->n;
-  int rc = copy_d(c, dst->pos, src->pos, sizeof(float4) * n);
-  if (!rc) rc = copy_d(c, dst->vel, src->vel, sizeof(float4) * n);
-  if (!rc) rc = copy_d(c, dst->vh, src->vh, sizeof(float4) * n);
-  if (!rc) rc = copy_d(c, dst->acc_s, src->acc_s, sizeof(float4) * n);
-  if (!rc) rc = copy_d(c, dst->rho_s, src->rho_s, sizeof(float) * n);
-  if (!rc) rc = copy_d(c, dst->prs_s, src->prs_s, sizeof(float) * n);
-  if (!rc) rc = copy_d(c, dst->key_b, src->key_b, sizeof(uint32_t) * n);
-  if (!rc) rc = copy_d(c, dst->slot, src->slot, sizeof(uint32_t) * n);
-  if (!rc && dst->ids && src->ids) rc = copy_d(c, dst->ids, src->ids, sizeof(uint32_t) * n);
+  int rc = copy_state_async(dst, src);
   if (rc) return rc;
-  if (dst->n != src->n) {
-    rc = resize_single(dst, src->n);
-    if (rc) return rc;
-  }
-  launch_aabb(dst);
-  dst->acc_zero = src->acc_zero;
-  dst->fields_canonical = src->fields_canonical;
-  dst->stage = -1;
-  rc = check_launch(c);
-  if (rc) return rc;
-  HIPCK(c, hipStreamSynchronize(c->stream));
+  HIPCK(dst->ctx, hipStreamSynchronize(dst->ctx->stream));
   return CLSPH_OK;
 }
 
@@ -349,7 +412,7 @@ int clsph_track_ids(clsph_state* st, int enable) {
     if (st->ids) {
       HIPCK(c, hipStreamSynchronize(c->stream));
       dfree(st->ids), dfree(st->ids_tmp);
-      st->graph_dirty = true;
+      st->graph_dirty = st->lgraph_dirty = true;
     }
     return CLSPH_OK;
   }
@@ -360,7 +423,7 @@ int clsph_track_ids(clsph_state* st, int enable) {
       dfree(st->ids), dfree(st->ids_tmp);
       return rc;
     }
-    st->graph_dirty = true;
+    st->graph_dirty = st->lgraph_dirty = true;
   }
   std::vector<uint32_t> iota(st->n_cap);
   for (uint32_t k = 0; k < st->n_cap; ++k) iota[k] = k;
@@ -382,27 +445,66 @@ int clsph_read_ids(clsph_state* st, uint32_t* ids, uint32_t n) {
 int clsph_learned_step(clsph_state* st, clsph_nn* m, uint32_t nsteps) {
   int rc = check_state(m, st);
   if (rc) return rc;
-  float g[3];
-  gravity(st, g);
-  for (uint32_t k = 0; k < nsteps; ++k) {
-    launch_grid_params(st);
-    launch_hash(st);
-    launch_sort(st);
-    launch_permute(st);
-    launch_cell_table(st);
-    launch_density(st);
-    launch_forces(st);
-    nn::forward_fused(m, st, st->acc_s, nullptr, g[0], g[1], g[2]);
-    launch_advect(st);
-    if (st->ids) launch_track_ids(st);
-    rc = check_launch(st->ctx);
-    if (rc) return rc;
-    st->acc_zero = true;
-    st->acc_learned = false;
-    st->fields_canonical = false;
-    st->stage = kStageAdvect;
-  }
+  if (st->n == 0 || nsteps == 0) return CLSPH_OK;
+  rc = ensure_learned_graph(st, m);
+  if (rc) return rc;
+  for (uint32_t k = 0; k < nsteps; ++k)
+    HIPCK(st->ctx, hipGraphLaunch(st->lgraph, st->ctx->stream));
+  st->acc_zero = true;
+  st->acc_learned = false;
+  st->fields_canonical = false;
+  st->stage = kStageAdvect;
   return device_errors(st);
+}
+
+int clsph_learned_run(clsph_state* learner, clsph_state* judge, clsph_nn* m,
+                      uint32_t macro_steps, uint32_t judge_every, float tol, uint32_t* judged,
+                      uint32_t* fallbacks, float* judge_err) {
+  int rc = check_state(m, learner);
+  if (!rc && judge_every) rc = check_state(m, judge);
+  if (rc) return rc;
+  if (judge_every && (judge == learner || judge->n_cap < learner->n || !judge->ids ||
+                      !learner->ids))
+    return fail(m->ctx, CLSPH_EINVAL,
+                "a judged run needs a second state of enough capacity and identities in both");
+  uint32_t nj = 0, nf = 0;
+  const float rho0 = learner->host_dps[0].p.fluid_density;
+  for (uint32_t k = 0; k < macro_steps; ++k) {
+    const bool judge_now = judge_every && k % judge_every == 0;
+    if (judge_now) {
+      rc = copy_state_async(judge, learner);
+      if (rc) return rc;
+    }
+    rc = ensure_learned_graph(learner, m);
+    if (rc) return rc;
+    HIPCK(m->ctx, hipGraphLaunch(learner->lgraph, m->ctx->stream));
+    learner->acc_zero = true;
+    learner->fields_canonical = false;
+    learner->stage = kStageAdvect;
+    if (!judge_now) continue;
+    rc = clsph_step(judge, nn::kMacro);
+    if (rc) return rc;
+    enqueue_density_stages(learner);
+    enqueue_density_stages(judge);
+    float res[2];
+    nn::launch_compare(m, learner, judge, res);
+    rc = check_launch(m->ctx);
+    if (rc) return rc;
+    HIPCK(m->ctx, hipStreamSynchronize(m->ctx->stream));
+    const float err = res[0] / rho0;
+    if (judge_err) judge_err[nj] = err;
+    ++nj;
+    if (!(err <= tol)) {
+      rc = copy_state_async(learner, judge);
+      if (rc) return rc;
+      ++nf;
+    }
+  }
+  if (judged) *judged = nj;
+  if (fallbacks) *fallbacks = nf;
+  rc = device_errors(learner);
+  if (!rc && judge_every) rc = device_errors(judge);
+  return rc;
 }
 
 int clsph_nn_compare_density(clsph_nn* m, clsph_state* a, clsph_state* b, float* max_rel,

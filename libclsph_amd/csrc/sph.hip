@@ -247,6 +247,10 @@ __device__ inline void force_pair(ForceSums& f, const ForceConsts& k, float4 ai,
   f.lap += bj.w * k.c_lap * t * (3.f * k.h2 - 7.f * d2);
 }
 
+// OVERFLOW_ONLY: only the particles whose neighbours overflow the list
+// (more than kMaxNeighbours), for callers that sum the listed ones
+// themselves.
+template <bool OVERFLOW_ONLY>
 __global__ __launch_bounds__(kMaxBlock) void k_forces(const float4* __restrict__ fa,
                                                       const float4* __restrict__ fb,
                                                       const float* __restrict__ rho,
@@ -260,6 +264,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_forces(const float4* __restrict__
   uint32_t scene;
   uint32_t i = particle_of(blocks, dps, scene);
   if (i == 0xFFFFFFFFu) return;
+  if (OVERFLOW_ONLY && nbr_count[i] <= kMaxNeighbours) return;
   const DevParams* dp = dps + scene;
   ForceConsts k;
   k.h = dp->p.h;
@@ -347,9 +352,18 @@ void launch_density(clsph_state* s) {
 
 void launch_forces(clsph_state* s) {
   if (s->n == 0) return;
-  hipLaunchKernelGGL(k_forces, dim3(s->nblocks_of(kStageForces)), dim3(s->bs_of(kStageForces)), 0,
-                     s->ctx->stream, s->fa, s->fb, s->rho_s, s->table, s->nbr, s->nbr_count,
-                     s->table_of(kStageForces), s->dps, s->bp, s->acc_s, s->n);
+  hipLaunchKernelGGL((k_forces<false>), dim3(s->nblocks_of(kStageForces)),
+                     dim3(s->bs_of(kStageForces)), 0, s->ctx->stream, s->fa, s->fb, s->rho_s,
+                     s->table, s->nbr, s->nbr_count, s->table_of(kStageForces), s->dps, s->bp,
+                     s->acc_s, s->n);
+}
+
+void launch_forces_overflow(clsph_state* s) {
+  if (s->n == 0) return;
+  hipLaunchKernelGGL((k_forces<true>), dim3(s->nblocks_of(kStageForces)),
+                     dim3(s->bs_of(kStageForces)), 0, s->ctx->stream, s->fa, s->fb, s->rho_s,
+                     s->table, s->nbr, s->nbr_count, s->table_of(kStageForces), s->dps, s->bp,
+                     s->acc_s, s->n);
 }
 
 }  // namespace clsph

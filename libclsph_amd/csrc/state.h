@@ -104,6 +104,12 @@ struct clsph_state {
 
   hipGraphExec_t graph = nullptr;  // one captured substep
   bool graph_dirty = true;
+  // One captured learned macro step (clsph_learned_step), for the network
+  // with serial lgraph_serial at lgraph_n particles; parameters are baked in.
+  hipGraphExec_t lgraph = nullptr;
+  bool lgraph_dirty = true;
+  uint64_t lgraph_serial = 0;
+  uint32_t lgraph_n = 0;
 
   // Last stage run (-1 after an upload) and whether the canonical
   // acceleration is the zero vector advection leaves behind.
@@ -136,6 +142,8 @@ uint32_t sort_tiles_for(uint32_t n);
 // sph.hip
 void launch_density(clsph_state* s);
 void launch_forces(clsph_state* s);
+// The forces stage for the particles whose neighbour list overflowed only.
+void launch_forces_overflow(clsph_state* s);
 // advect.hip
 void launch_advect(clsph_state* s);
 

@@ -189,29 +189,16 @@ class Run:
 def run_learned(ctx, learner, teacher, net, macro_steps, dt, judge_every=8, tol=0.01):
     """Advances `learner` (parameters at dt_macro = 4 dt) by `macro_steps`
     macro steps with the network, judged every `judge_every` macro steps by
-    4 plain substeps of `teacher` (parameters at dt) from the same start.
-    Both states must track identities."""
+    4 plain substeps of `teacher` (parameters at dt) from the same start
+    (clsph_learned_run). Both states must track identities."""
     r = Run()
     ctx.sync()
     t0 = time.perf_counter()
-    for k in range(macro_steps):
-        if judge_every and k % judge_every == 0:
-            teacher.copy_from(learner)
-            learner.learned_step(net, 1)
-            teacher.step(MACRO)
-            density_stages(learner)
-            density_stages(teacher)
-            mx, _ = net.compare_density(learner, teacher)
-            r.judged += 1
-            r.judge_err.append(mx)
-            if mx > tol:
-                learner.copy_from(teacher)
-                r.fallbacks += 1
-        else:
-            learner.learned_step(net, 1)
-        r.macro_steps += 1
+    r.judged, r.fallbacks, r.judge_err = net.learned_run(learner, teacher, macro_steps,
+                                                         judge_every, tol)
     ctx.sync()
     r.wall_s = time.perf_counter() - t0
+    r.macro_steps = macro_steps
     r.sim_s = macro_steps * MACRO * dt
     return r
 

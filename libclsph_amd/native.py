@@ -93,6 +93,7 @@ def _load():
         "clsph_nn_get_params": (_i32, [_vp, _vp, _u32]),
         "clsph_nn_forward": (_i32, [_vp, _vp, _vp]),
         "clsph_nn_infer": (_i32, [_vp, _vp]),
+        "clsph_learned_run": (_i32, [_vp, _vp, _vp, _u32, _u32, ctypes.c_float, _vp, _vp, _vp]),
         "clsph_nn_profile": (_i32, [_vp, _vp, _u32, _vp, _vp]),
         "clsph_nn_read_graph": (_i32, [_vp, _vp, _vp, _u32, _u32]),
         "clsph_nn_read_output": (_i32, [_vp, _vp, _u32]),
@@ -464,6 +465,17 @@ class Network:
     def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
         self.ctx.check(lib.clsph_nn_adam(self.h, ctypes.c_float(lr), ctypes.c_float(beta1),
                                          ctypes.c_float(beta2), ctypes.c_float(eps)))
+
+    def learned_run(self, learner, judge, macro_steps, judge_every=8, tol=0.01):
+        """Judged learned run (clsph_learned_run); returns (judged,
+        fallbacks, [error of each judged step])."""
+        nj = (macro_steps + judge_every - 1) // judge_every if judge_every else 0
+        errs = np.zeros(max(nj, 1), dtype=np.float32)
+        counts = np.zeros(2, dtype=np.uint32)
+        self.ctx.check(lib.clsph_learned_run(
+            learner.h, judge.h if judge is not None else None, self.h, macro_steps,
+            judge_every, ctypes.c_float(tol), _ptr(counts[0:1]), _ptr(counts[1:2]), _ptr(errs)))
+        return int(counts[0]), int(counts[1]), [float(e) for e in errs[:int(counts[0])]]
 
     def compare_density(self, a, b):
         """(max, mean) of |rho_a - rho_b| / rho0 over the same particles."""

@@ -7,7 +7,8 @@
   gradients vs the FP64 oracle's backward     : per tensor, relative L2 <= 5 %
   fused inference vs the training forward     : within 1 % of scale, rms 1e-3
   training targets, identities                : closed form, 1e-5
-  zero network                                : the plain substep, bit for bit
+  zero network                                : the forces stage, bit for bit;
+                                                the plain substep within 1e-4 max|a| dt
 """
 import numpy as np
 import pytest
@@ -201,9 +202,23 @@ def test_target_and_identities(ctx):
     assert np.abs(x1 - x0).max() < 4 * float(sc.params["time_delta"]) * 5.0
 
 
-def test_zero_network_is_the_plain_substep(ctx):
-    """With every parameter zero the correction is exactly zero: a learned
-    step equals the plain substep bit for bit."""
+def test_zero_network_leaves_the_forces_stage_unchanged(ctx):
+    """With every parameter zero the correction is exactly zero: inference
+    after the forces stage leaves its accelerations bit for bit."""
+    sc, st = settled_state(ctx, n=3000, steps=200, seed=21)
+    a0 = internal_sorted(st)[0]["acceleration"].copy()
+    net = native.Network(ctx, st.n)
+    net.set_params(np.zeros(net.count, np.float32))
+    net.infer(st)
+    assert np.array_equal(internal_sorted(st)[0]["acceleration"], a0)
+
+
+def test_learned_step_sums_the_forces_stage(ctx):
+    """The learned step sums the forces stage's pair terms inside the network
+    kernel (the forces kernel runs only for particles whose neighbour list
+    overflowed). With a zero network one learned step is the plain substep
+    up to the order of that summation: velocities agree within 1e-4 of
+    max |a| dt, the forces parity bound."""
     sc = scenes.dam_break(3000, seed=2)
     a = native.State(ctx, sc.n)
     b = native.State(ctx, sc.n)
@@ -211,14 +226,22 @@ def test_zero_network_is_the_plain_substep(ctx):
         s.set_params(sc.params, sc.terms)
         s.scene(sc.normals, sc.vertices, sc.indices)
         s.upload(sc.particles)
-        s.step(20)
+        s.step(1500)
+    for name in ("grid", "hash", "sort", "cell_table", "density", "forces"):
+        b.stage(name)
+    amax = float(np.abs(internal_sorted(b)[0]["acceleration"][:, :3]).max())
+    b.stage("advect")
     net = native.Network(ctx, sc.n)
     net.set_params(np.zeros(net.count, np.float32))
-    a.learned_step(net, 3)
-    b.step(3)
+    a.learned_step(net, 1)
     pa, pb = a.download(), b.download()
-    for f in ("position", "velocity", "intermediate_velocity"):
-        assert np.array_equal(pa[f], pb[f]), f
+    dt = float(sc.params["time_delta"])
+    dv = np.abs(pa["intermediate_velocity"][:, :3] - pb["intermediate_velocity"][:, :3]).max()
+    dx = np.abs(pa["position"][:, :3] - pb["position"][:, :3]).max()
+    print(f"max |a| {amax:.3e}, velocity difference {dv:.3e} (bound {1e-4 * amax * dt:.3e}), "
+          f"position difference {dx:.3e}", flush=True)
+    assert dv <= 1e-4 * amax * dt
+    assert dx <= 1e-4 * amax * dt * dt + 1e-9
 
 
 def test_compare_density_of_a_copy_is_zero(ctx):
