@@ -113,7 +113,7 @@ def main():
                     help="also track particle identities each substep; its copy kernel "
                          "(k_copy_u32, 4 n bytes read and written) calibrates HBM counters")
     ap.add_argument("--learned-macro-steps", type=int, default=100)
-    ap.add_argument("--judge-every", type=int, default=8,
+    ap.add_argument("--judge-every", type=int, default=16,
                     help="learned macro steps between checks against the plain method")
     args = ap.parse_args()
 
@@ -247,7 +247,7 @@ NN_NODE_FLOP = 2.0 * (38 * 64 + 64 * 64 + 64 * 3)
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
-def bench_learned(ctx, plain, scene, plain_ms_per_step, macro_steps, judge_every=8, tol=0.01):
+def bench_learned(ctx, plain, scene, plain_ms_per_step, macro_steps, judge_every=16, tol=0.01):
     """The learned surrogate (trained weights shipped in-tree) against the
     plain method over the same simulated interval from the same state: the
     plain substeps timed over that interval, then the learned macro steps
