@@ -240,9 +240,9 @@ def main():
 # Learned surrogate: multiply-adds per evaluated edge and round in the fused
 # edge kernel (layer 1: 16 inputs -> 128; layer 2: 128 -> 128).
 # Matrix work of one surrogate evaluation over the used inputs: the edge
-# layer (22 inputs -> 32) per edge and the node MLP (38 -> 64 -> 64 -> 3)
+# layer (27 inputs -> 32) per edge and the node MLP (38 -> 64 -> 64 -> 3)
 # per particle, 2 flops per multiply-add.
-NN_EDGE_FLOP = 2.0 * 22 * 32
+NN_EDGE_FLOP = 2.0 * 27 * 32
 NN_NODE_FLOP = 2.0 * (38 * 64 + 64 * 64 + 64 * 3)
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 

@@ -8,7 +8,9 @@
 //   nu_i  = [rho_i / rho0 - 1, asinh(p_i / K) / 4]
 //   e_ij  = [(x_i - x_j) / h, (v_i - v_j) / V, q, (1 - q^2)^3, nu_i, nu_j,
 //            pressure and viscosity pair accelerations / A, colour-field
-//            gradient term * h and laplacian term * h^2, 0 x 10]
+//            gradient term * h and laplacian term * h^2, the rate of the
+//            pressure pair acceleration along the pair's motion, the
+//            density-rate pair term, (x_i - x_j).(v_i - v_j), 0 x 5]
 //   edges: S_i = bf16(sum_j relu(e_ij W1 + b1) / C)                (32 wide)
 //   node:  u_i = [S_i, nu_i, (a0_i - g) / A, deg_i / C, 0 x 26]     (64 wide)
 //          out_i = 3-layer MLP of u_i, SiLU between layers (64 -> 64 -> 64 -> 32)
@@ -26,8 +28,8 @@ namespace nn {
 
 constexpr int kEdgeW = 32;   // edge layer width
 constexpr int kNodeW = 64;   // node MLP width
-constexpr int kFeat = 32;    // edge inputs (22 used)
-constexpr int kFeatUsed = 22;
+constexpr int kFeat = 32;    // edge inputs (27 used)
+constexpr int kFeatUsed = 27;
 constexpr int kIn = 64;      // node MLP input (38 used)
 constexpr int kOut = 32;     // network output (3 used)
 constexpr float kAggC = 32.f;

@@ -257,14 +257,16 @@ __global__ __launch_bounds__(256) void k_graph(const uint4* __restrict__ nbr,
 
 // Constants of the edge inputs (one scene).
 struct EdgeConsts {
-  float h, inv_h, rho0, K, m, mu_over_m, c_spiky, c_visc, c_grad, c_lap;
+  float h, h2, inv_h, rho0, inv_rho0, K, m, mu_over_m, c_spiky, c_visc, c_grad, c_lap;
 };
 
 inline EdgeConsts edge_consts(const DevParams& d) {
   EdgeConsts k;
   k.h = d.p.h;
+  k.h2 = d.p.h * d.p.h;
   k.inv_h = 1.f / d.p.h;
   k.rho0 = d.p.fluid_density;
+  k.inv_rho0 = 1.f / d.p.fluid_density;
   k.K = d.p.K;
   k.m = d.p.particle_mass;
   k.mu_over_m = d.p.dynamic_viscosity / d.p.particle_mass;
@@ -309,7 +311,7 @@ __device__ __attribute__((always_inline)) inline void pair_terms(PairSums& f, co
   float dx = ai.x - aj.x, dy = ai.y - aj.y, dz = ai.z - aj.z;
   float d2 = dx * dx + dy * dy + dz * dz;
   float r = __builtin_amdgcn_sqrtf(d2);
-  float t = ft.h2 - d2;
+  float t = k.h2 - d2;
   float hr = k.h - r;
   float ps = (aj.w + ai.w) * k.m;
   const bool coincident = r < 1e-7f;
@@ -326,28 +328,33 @@ __device__ __attribute__((always_inline)) inline void pair_terms(PairSums& f, co
   f.nx += gw * dx;
   f.ny += gw * dy;
   f.nz += gw * dz;
-  f.lap += bj.w * k.c_lap * t * (3.f * ft.h2 - 7.f * d2) * w;
+  f.lap += bj.w * k.c_lap * t * (3.f * k.h2 - 7.f * d2) * w;
 }
 
-// The 32 edge inputs of edge (i <- j) in float32 (22 used, 10 zero), from
+// The 32 edge inputs of edge (i <- j) in float32 (27 used, 5 zero), from
 // the density stage's per-particle force inputs a = (x, p / rho^2) and
 // b = (v, m / rho) and the node inputs nu. The pair terms are the ones the
 // forces stage sums (kernels/forces.cl:60-101): pressure and viscosity
 // accelerations in units of A, the colour-field gradient term times h and
-// its laplacian term times h^2.
+// its laplacian term times h^2. Then three rates along the pair's motion
+// (x = x_i - x_j, v = v_i - v_j): the time derivative of the pressure pair
+// acceleration at fixed pressures, -ps c_spiky [(h - r)^2 / r v -
+// (x.v) (h - r)(h + r) / r^3 x], times h / (V A); the density-rate pair term
+// m c_grad (h^2 - r^2)^2 (x.v) times 1000 h / (rho0 V); and x.v times
+// 10 / (h V).
 __device__ inline void edge_features(float4 ai, float4 bi, float4 aj, float4 bj, float2 ni,
                                      float2 nj, const EdgeConsts& k, float f[kFeat]) {
   constexpr float inv_a = 1.f / kAccScale, inv_v = 1.f / kVelScale;
   float dx = ai.x - aj.x, dy = ai.y - aj.y, dz = ai.z - aj.z;
   float d2 = dx * dx + dy * dy + dz * dz;
-  float r = sqrtf(d2);
+  float r = __builtin_amdgcn_sqrtf(d2);  // the forces kernel's square root
   float q = r * k.inv_h;
   float w = 1.f - q * q;
   f[0] = dx * k.inv_h, f[1] = dy * k.inv_h, f[2] = dz * k.inv_h;
   f[3] = (bi.x - bj.x) * inv_v, f[4] = (bi.y - bj.y) * inv_v, f[5] = (bi.z - bj.z) * inv_v;
   f[6] = q, f[7] = w * w * w;
   f[8] = ni.x, f[9] = ni.y, f[10] = nj.x, f[11] = nj.y;
-  const float hr = k.h - r, t = k.h * k.h - d2;
+  const float hr = k.h - r, t = k.h2 - d2;
   const float ps = (ai.w + aj.w) * k.m;
   const float g = r >= 1e-7f ? ps * k.c_spiky * hr * hr * __builtin_amdgcn_rcpf(r) : 0.f;
   f[12] = -g * dx * inv_a, f[13] = -g * dy * inv_a, f[14] = -g * dz * inv_a;
@@ -356,8 +363,21 @@ __device__ inline void edge_features(float4 ai, float4 bi, float4 aj, float4 bj,
   f[15] = (bj.x - bi.x) * vw, f[16] = (bj.y - bi.y) * vw, f[17] = (bj.z - bi.z) * vw;
   const float gw = bj.w * k.c_grad * t * t * k.h;
   f[18] = gw * dx, f[19] = gw * dy, f[20] = gw * dz;
-  f[21] = bj.w * k.c_lap * t * (3.f * k.h * k.h - 7.f * d2) * k.h * k.h;
-  for (int c = 22; c < kFeat; ++c) f[c] = 0.f;
+  f[21] = bj.w * k.c_lap * t * (3.f * k.h2 - 7.f * d2) * k.h2;
+  const float vx = bi.x - bj.x, vy = bi.y - bj.y, vz = bi.z - bj.z;
+  const float xv = dx * vx + dy * vy + dz * vz;
+  float ra = 0.f, rb = 0.f;
+  if (r >= 1e-7f) {
+    const float ir = __builtin_amdgcn_rcpf(r);
+    ra = hr * hr * ir;
+    rb = xv * hr * (k.h + r) * ir * ir * ir;
+  }
+  const float cr = -ps * k.c_spiky * k.h * inv_v * inv_a;
+  f[22] = cr * (ra * vx - rb * dx), f[23] = cr * (ra * vy - rb * dy);
+  f[24] = cr * (ra * vz - rb * dz);
+  f[25] = k.m * k.c_grad * t * t * xv * (1000.f * k.h * inv_v) * k.inv_rho0;
+  f[26] = xv * (10.f * inv_v) * k.inv_h;
+  for (int c = kFeatUsed; c < kFeat; ++c) f[c] = 0.f;
 }
 
 // nu_i = [rho_i / rho0 - 1, asinh(p_i / K) / 4] for every particle.
@@ -400,12 +420,11 @@ __global__ __launch_bounds__(256) void k_edge_inputs(
 // A lane holds z for hidden units 16 ob + 4 g + r (r = 0..3) of its column.
 //
 // Edge inputs are computed once per edge. In each round of 4 steps, lane
-// (g, c) computes the inputs of particle c's edge 4 k + g and stores inputs
-// 0..23 as bf16 in the wave's LDS block; the B operand of step 4 k + s for
-// lane (g, c) is inputs 8 g .. 8 g + 7 of edge 4 k + s of particle c (lane
-// group 3 reads group 2's inputs, which meet the zero rows 24..31 of W1^T).
-// Per particle column the block holds 4 edges x 48 bytes plus 16 bytes of
-// padding: 208 bytes = 52 dwords, so the 16 columns of one read start on 16
+// (g, c) computes the inputs of particle c's edge 4 k + g and stores its 32
+// inputs as bf16 in the wave's LDS block; the B operand of step 4 k + s for
+// lane (g, c) is inputs 8 g .. 8 g + 7 of edge 4 k + s of particle c. Per
+// particle column the block holds 4 edges x 64 bytes plus 16 bytes of
+// padding: 272 bytes = 68 dwords, so the 16 columns of one read start on 16
 // distinct multiples of 4 banks.
 //
 // The node MLP follows in registers: the neighbour sums are the B operand of
@@ -419,14 +438,16 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTileP = 16;                 // particles per tile (MFMA columns)
 constexpr int kSurWaves = 8;               // waves per work-group
-constexpr int kColBytes = 4 * 48 + 16;     // LDS bytes per particle column
+constexpr int kColBytes = 4 * 64 + 16;     // LDS bytes per particle column
 constexpr int kWaveLds = kTileP * kColBytes;
 
+// Eight floats to bf16 (round to nearest even), as one vector conversion so
+// that each pair becomes one two-operand v_cvt_pk_bf16_f32.
+typedef float f32x8 __attribute__((ext_vector_type(8)));
+
 __device__ __attribute__((always_inline)) inline u32x4 pack8(const float v[8]) {
-  u32x4 r;
-  for (int t = 0; t < 4; ++t)
-    r[t] = (uint32_t)to_bf16(v[2 * t]) | ((uint32_t)to_bf16(v[2 * t + 1]) << 16);
-  return r;
+  const f32x8 x = {v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
+  return __builtin_bit_cast(u32x4, __builtin_convertvector(x, bf16x8));
 }
 
 // Stages W (row-major, rows = inputs, ld = outputs) for k-steps
@@ -525,18 +546,17 @@ __device__ __attribute__((always_inline)) inline void edge_round(
   float f[kFeat];
   edge_features(ai, bi, ej.a, ej.b, ni, ej.nu, kc, f);
   uint8_t* col = wlds + c * kColBytes;
-  for (int q = 0; q < 3; ++q) {
+  for (int q = 0; q < 4; ++q) {
     u32x4 v = pack8(f + 8 * q);
     if (MASKED && !ej.mine) v = u32x4{0u, 0u, 0u, 0u};
-    *(u32x4*)(col + g * 48 + 16 * q) = v;
+    *(u32x4*)(col + g * 64 + 16 * q) = v;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const int gr = g < 3 ? g : 2;
 #pragma unroll
   for (int st = 0; st < 4; ++st) {
-    const bf16x8 eb = __builtin_bit_cast(bf16x8, *(const u32x4*)(col + st * 48 + gr * 16));
+    const bf16x8 eb = __builtin_bit_cast(bf16x8, *(const u32x4*)(col + st * 64 + g * 16));
     const bool live = !MASKED || 4 * k + st < deg;
 #pragma unroll
     for (int ob = 0; ob < kEdgeW / 16; ++ob) {
@@ -544,7 +564,9 @@ __device__ __attribute__((always_inline)) inline void edge_round(
       if (MASKED && !live) c0 = f32x4{0.f, 0.f, 0.f, 0.f};
       f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
           __builtin_bit_cast(bf16x8, w1s[ob * 64 + lane]), eb, c0, 0, 0, 0);
-      for (int r = 0; r < 4; ++r) s[ob][r] += fmaxf(z[r], 0.f);
+      // s + relu(z) as max(s + z, s): the same value and rounding, and the
+      // operands of the max are canonical (one instruction, not two)
+      for (int r = 0; r < 4; ++r) s[ob][r] = fmaxf(s[ob][r] + z[r], s[ob][r]);
     }
   }
   // the next round's stores come after this round's reads

@@ -49,7 +49,7 @@ FORCES_STAGES = DENSITY_STAGES + ("forces",)
 # (name, shape, used input rows) in storage order, row-major inputs x outputs.
 EW, NW, FEAT, IN, OUT = 32, 64, 32, 64, 32
 LAYOUT = [
-    ("e.w1", (FEAT, EW), 22), ("e.b1", (EW,), 0),
+    ("e.w1", (FEAT, EW), 27), ("e.b1", (EW,), 0),
     ("n.w1", (IN, NW), 38), ("n.b1", (NW,), 0),
     ("n.w2", (NW, NW), NW), ("n.b2", (NW,), 0),
     ("n.w3", (NW, OUT), NW), ("n.b3", (OUT,), 0),

@@ -16,7 +16,10 @@ passing over the particles' neighbour graph (r < h, self excluded, at most
                         nu_i, nu_j,                     q = |x_i - x_j| / h
                         pressure and viscosity pair accelerations / A,
                         colour-gradient pair term * h, colour-laplacian term * h^2,
-                        0 x 10]
+                        d/dt of the pressure pair acceleration at fixed
+                        pressures * h / (V A), the density-rate pair term
+                        * 1000 h / (rho0 V), (x_i - x_j).(v_i - v_j) * 10 / (h V),
+                        0 x 5]
                 the pair terms are those the forces kernel sums
                 (kernels/forces.cl:60-101), from positions, velocities,
                 densities and pressures only
@@ -39,8 +42,8 @@ import numpy as np
 
 EDGE_W = 32        # edge layer width
 NODE_W = 64        # node MLP width
-FEAT = 32          # edge input features (22 used, 10 zero)
-FEAT_USED = 22
+FEAT = 32          # edge input features (27 used, 5 zero)
+FEAT_USED = 27
 NODE_IN = 64       # node MLP input: S (32) + nu (2) + a0 (3) + deg (1) + zeros
 NODE_IN_USED = 38
 OUT = 32           # network output (3 used)
@@ -115,8 +118,8 @@ def node_inputs(rho, prs, rho0, K):
     return nu
 
 
-def edge_inputs(x, v, nu, src, dst, h, rho=None, prs=None, consts=None):
-    """Edge inputs; the pair terms (columns 12..21) need rho, prs and
+def edge_inputs(x, v, nu, src, dst, h, rho=None, prs=None, consts=None, rho0=1000.0):
+    """Edge inputs; the pair terms (columns 12..26) need rho, prs and
     consts = (particle_mass, dynamic_viscosity, spiky, viscosity,
     poly_6_gradient, poly_6_laplacian) and are zero without them."""
     x = np.asarray(x, np.float64)
@@ -149,6 +152,19 @@ def edge_inputs(x, v, nu, src, dst, h, rho=None, prs=None, consts=None):
         w = m / rho[src]
         e[:, 18:21] = (w * c_grad * t * t * h)[:, None] * d
         e[:, 21] = w * c_lap * t * (3 * h * h - 7 * r * r) * h * h
+        # rates along the pair's motion (x = x_i - x_j, v = v_i - v_j): the
+        # time derivative of the pressure pair acceleration at fixed
+        # pressures, -ps c_spiky [(h - r)^2 / r v - (x.v)(h - r)(h + r) / r^3 x]
+        dv = v[dst] - v[src]
+        xv = (d * dv).sum(1)
+        rs = np.maximum(r, 1e-30)
+        ra = np.where(r >= 1e-7, hr * hr / rs, 0.0)
+        rb = np.where(r >= 1e-7, xv * hr * (h + r) / rs ** 3, 0.0)
+        cr = -ps * c_spiky * h / (VEL_SCALE * ACC_SCALE)
+        e[:, 22:25] = cr[:, None] * (ra[:, None] * dv - rb[:, None] * d)
+        # the density-rate pair term m c_grad (h^2 - r^2)^2 (x.v), x 1000 h / (rho0 V)
+        e[:, 25] = m * c_grad * t * t * xv * 1000.0 * h / (rho0 * VEL_SCALE)
+        e[:, 26] = xv * 10.0 / (h * VEL_SCALE)
     return e
 
 
@@ -180,7 +196,7 @@ def forward(flat, x, v, rho, prs, acc0, src, dst, deg, h, rho0, K, g, consts=Non
     R = bf16 if bf16_round else (lambda a: np.asarray(a, np.float64))
     n = len(rho)
     nu = node_inputs(rho, prs, rho0, K)
-    e = R(edge_inputs(x, v, nu, src, dst, h, rho, prs, consts))
+    e = R(edge_inputs(x, v, nu, src, dst, h, rho, prs, consts, rho0))
     z = e @ R(P["e.w1"]) + P["e.b1"]
     S = R(segsum(relu(z), dst, n) / AGG_C)
     u = R(node_vector(S, nu, acc0, deg, g))

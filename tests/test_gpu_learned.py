@@ -6,8 +6,11 @@ On a dam-break variant the training never saw, from one common state:
   (c) the learned method at 4 dt            (the plain substep plus the network's correction)
 Densities of the same particles are compared at the matched time. The
 learned run must stay finite with its density error against (a) inside the
-judge's bound, and the test prints what it gains over (b). A separate case
-compares both GPU runs with an FP64 trajectory of the CPU oracle.
+judge's bound, and the test prints what it gains over (b). On states of a
+held-out variant the network's squared error against its training target
+(the mean forces-stage acceleration over the next 4 substeps) must be below
+that of the method without it. A separate case compares both GPU runs with
+an FP64 trajectory of the CPU oracle.
 """
 import os
 
@@ -85,6 +88,44 @@ def test_learned_gain_over_plain_at_large_step(ctx):
     assert err["learned_4dt"].mean() <= 0.01
     print(f"mean density error, plain at 4 dt / learned at 4 dt: "
           f"{err['plain_4dt'].mean() / max(err['learned_4dt'].mean(), 1e-12):.2f}x", flush=True)
+
+
+def held_out_loss(ctx, net, seed=100007, samples=8, gap=40, warm=300):
+    """Mean squared correction error of the network on states of a held-out
+    dam-break variant, and that of the zero network (the plain step at 4 dt),
+    over the same targets (in units of R^2)."""
+    sc = learned.slab_variant(seed)
+    teacher = learned.make_state(ctx, sc)
+    learner = learned.make_state(ctx, sc)
+    teacher.upload(sc.particles)
+    teacher.track_ids(True)
+    teacher.step(warm)
+    got, zero = 0.0, 0.0
+    for _ in range(samples):
+        learner.copy_from(teacher)
+        learned.forces_stages(learner)
+        teacher.track_ids(True)
+        net.target_mean_acc(learner, teacher, learned.MACRO)
+        net.forward(learner)
+        got += net.backward()
+        t = net.target(learner.n).astype(np.float64)
+        zero += float((t * t).sum(1).mean())
+        teacher.step(gap - learned.MACRO)
+    return got / samples, zero / samples
+
+
+def test_network_reduces_the_correction_error_on_held_out_states(ctx):
+    """What the network is trained for: the mean forces-stage acceleration
+    over the next 4 substeps. On states of a dam-break variant the training
+    never saw, its squared error against that target is below the zero
+    network's, which is the plain method's at the 4 dt step."""
+    sc = learned.slab_variant(100007)
+    net = native.Network(ctx, sc.n, e_cap=80 * sc.n)
+    net.set_params(np.load(learned.WEIGHTS))
+    got, zero = held_out_loss(ctx, net)
+    print(f"held-out correction error: network {got:.4e}, zero network {zero:.4e}, "
+          f"ratio {got / zero:.3f}", flush=True)
+    assert got < 0.9 * zero
 
 
 def test_judged_run_counts_fallbacks(ctx):

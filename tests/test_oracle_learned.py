@@ -42,7 +42,7 @@ def test_param_layout():
     assert L.param_count() == 11456
     flat = L.init_params(0)
     P = L.unpack(flat)
-    assert np.all(P["e.w1"][22:] == 0) and np.all(P["n.w1"][38:] == 0)
+    assert np.all(P["e.w1"][27:] == 0) and np.all(P["n.w1"][38:] == 0)
     assert np.all(P["n.w3"][:, 3:] == 0)
     assert np.array_equal(L.pack(P), flat)
 
@@ -77,9 +77,9 @@ def test_padded_parameters_get_zero_gradient():
     target = np.ones((len(g["rho"]), 3))
     out, c = L.forward(flat, keep=True, **g)
     G = L.unpack(L.backward(flat, c, target, g["src"], g["dst"], g["deg"]))
-    assert np.all(G["e.w1"][22:] == 0) and np.all(G["n.w1"][38:] == 0)
+    assert np.all(G["e.w1"][27:] == 0) and np.all(G["n.w1"][38:] == 0)
     assert np.all(G["n.w3"][:, 3:] == 0) and np.all(G["n.b3"][3:] == 0)
-    assert np.abs(G["e.w1"][:22]).max() > 0 and np.abs(G["n.w1"][:38]).max() > 0
+    assert np.abs(G["e.w1"][:27]).max() > 0 and np.abs(G["n.w1"][:38]).max() > 0
 
 
 def test_literal_per_particle_evaluation():
@@ -110,8 +110,14 @@ def test_literal_per_particle_evaluation():
             t = h * h - r * r
             grad = (m / g["rho"][j]) * c_grad * t * t * h * d
             lap = (m / g["rho"][j]) * c_lap * t * (3 * h * h - 7 * r * r) * h * h
-            e = np.concatenate([d / h, g["v"][i] - g["v"][j], [q, (1 - q * q) ** 3],
-                                nu_i, nu_j, pres, visc, grad, [lap], np.zeros(10)])
+            dv = g["v"][i] - g["v"][j]
+            xv = d @ dv
+            rate = -ps * c_spiky * ((h - r) ** 2 / r * dv - xv * (h - r) * (h + r) / r ** 3 * d) \
+                * h / (L.VEL_SCALE * L.ACC_SCALE)
+            rho_rate = m * c_grad * t * t * xv * 1000.0 * h / (g["rho0"] * L.VEL_SCALE)
+            e = np.concatenate([d / h, dv, [q, (1 - q * q) ** 3], nu_i, nu_j, pres, visc, grad,
+                                [lap], rate, [rho_rate, xv * 10.0 / (h * L.VEL_SCALE)],
+                                np.zeros(5)])
             S += np.maximum(e @ P["e.w1"] + P["e.b1"], 0.0) / L.AGG_C
         u = np.concatenate([S, nu_i, (g["acc0"][i] - g["g"]) / L.ACC_SCALE, [deg / L.AGG_C],
                             np.zeros(26)])
@@ -119,6 +125,39 @@ def test_literal_per_particle_evaluation():
         y2 = silu(y1 @ P["n.w2"] + P["n.b2"])
         want = y2 @ P["n.w3"] + P["n.b3"]
         assert np.allclose(out[i], want, rtol=1e-11, atol=1e-12), i
+
+
+def test_motion_rate_inputs_are_time_derivatives():
+    """Inputs 22..25 are rates along the pair's motion: moving both particles
+    along their velocities, central differences in time of the pressure pair
+    acceleration (at fixed pressures) and of the pair's poly6 density term
+    give them."""
+    g = tiny_graph(seed=13)
+    h = g["h"]
+    m, mu, c_spiky, c_visc, c_grad, c_lap = g["consts"]
+    poly6 = 315.0 / (64.0 * np.pi * h ** 9)
+    nu = L.node_inputs(g["rho"], g["prs"], g["rho0"], g["K"])
+    e = L.edge_inputs(g["x"], g["v"], nu, g["src"], g["dst"], h, g["rho"], g["prs"],
+                      g["consts"], g["rho0"])
+    eps = 1e-7
+    for k in range(len(g["src"])):
+        i, j = g["dst"][k], g["src"][k]
+        ps = (g["prs"][i] / g["rho"][i] ** 2 + g["prs"][j] / g["rho"][j] ** 2) * m
+
+        def pres(tt):
+            d = (g["x"][i] + tt * g["v"][i]) - (g["x"][j] + tt * g["v"][j])
+            r = np.sqrt(d @ d)
+            return -ps * c_spiky * (h - r) ** 2 / r * d
+
+        def dens(tt):
+            d = (g["x"][i] + tt * g["v"][i]) - (g["x"][j] + tt * g["v"][j])
+            return m * poly6 * (h * h - d @ d) ** 3
+
+        rate = (pres(eps) - pres(-eps)) / (2 * eps) * h / (L.VEL_SCALE * L.ACC_SCALE)
+        rho_rate = (dens(eps) - dens(-eps)) / (2 * eps) * 1000.0 * h / (g["rho0"] * L.VEL_SCALE)
+        scale = np.abs(rate).max() + 1e-12
+        assert np.allclose(e[k, 22:25], rate, rtol=1e-5, atol=1e-6 * scale), k
+        assert np.isclose(e[k, 25], rho_rate, rtol=1e-5, atol=1e-9), k
 
 
 def test_zero_network_returns_the_forces_stage_acceleration():
