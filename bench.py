@@ -325,6 +325,7 @@ def bench_learned(ctx, plain, scene, plain_ms_per_step, macro_steps, judge_every
         "judged": r.judged,
         "fallbacks": r.fallbacks,
         "judge_max_density_err": max(r.judge_err) if r.judge_err else None,
+        "judge_mean_density_err": max(r.judge_mean) if r.judge_mean else None,
         "density_err_vs_plain": {"max": err_max, "mean": err_mean},
         "plain_4dt_density_err_vs_plain": {"max": big_max, "mean": big_mean},
         "literal_max_abs_rho_over_rho0_minus_1": lit,

@@ -244,11 +244,12 @@ int clsph_learned_step(clsph_state* st, clsph_nn* nn, uint32_t nsteps);
  * starts from the learner's particles and runs 4 plain substeps, both get
  * their densities, and when max |rho - rho_judge| / fluid_density exceeds
  * `tol` the learner takes the judge's particles (a fallback). Writes the
- * counts and each judged step's error (judge_err: one float per judged step,
- * optional). One host synchronisation per judged step. */
+ * counts and, per judged step (one float each, optional), the maximum and
+ * the mean of |rho - rho_judge| / fluid_density. One host synchronisation
+ * per judged step. No counterpart in the reference. */
 int clsph_learned_run(clsph_state* learner, clsph_state* judge, clsph_nn* nn,
                       uint32_t macro_steps, uint32_t judge_every, float tol, uint32_t* judged,
-                      uint32_t* fallbacks, float* judge_err);
+                      uint32_t* fallbacks, float* judge_err, float* judge_mean);
 /* Density difference of the same particles in two states after both ran
  * their density stage (identities tracked in both): max and mean of
  * |rho_a - rho_b| / fluid_density. */

@@ -459,7 +459,7 @@ int clsph_learned_step(clsph_state* st, clsph_nn* m, uint32_t nsteps) {
 
 int clsph_learned_run(clsph_state* learner, clsph_state* judge, clsph_nn* m,
                       uint32_t macro_steps, uint32_t judge_every, float tol, uint32_t* judged,
-                      uint32_t* fallbacks, float* judge_err) {
+                      uint32_t* fallbacks, float* judge_err, float* judge_mean) {
   int rc = check_state(m, learner);
   if (!rc && judge_every) rc = check_state(m, judge);
   if (rc) return rc;
@@ -493,6 +493,7 @@ int clsph_learned_run(clsph_state* learner, clsph_state* judge, clsph_nn* m,
     HIPCK(m->ctx, hipStreamSynchronize(m->ctx->stream));
     const float err = res[0] / rho0;
     if (judge_err) judge_err[nj] = err;
+    if (judge_mean) judge_mean[nj] = res[1] / rho0 / (float)(learner->n ? learner->n : 1);
     ++nj;
     if (!(err <= tol)) {
       rc = copy_state_async(learner, judge);

@@ -182,6 +182,7 @@ class Run:
         self.judged = 0
         self.fallbacks = 0
         self.judge_err = []
+        self.judge_mean = []
         self.wall_s = 0.0
         self.sim_s = 0.0
 
@@ -194,8 +195,8 @@ def run_learned(ctx, learner, teacher, net, macro_steps, dt, judge_every=8, tol=
     r = Run()
     ctx.sync()
     t0 = time.perf_counter()
-    r.judged, r.fallbacks, r.judge_err = net.learned_run(learner, teacher, macro_steps,
-                                                         judge_every, tol)
+    r.judged, r.fallbacks, r.judge_err, r.judge_mean = net.learned_run(
+        learner, teacher, macro_steps, judge_every, tol)
     ctx.sync()
     r.wall_s = time.perf_counter() - t0
     r.macro_steps = macro_steps

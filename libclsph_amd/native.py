@@ -93,7 +93,8 @@ def _load():
         "clsph_nn_get_params": (_i32, [_vp, _vp, _u32]),
         "clsph_nn_forward": (_i32, [_vp, _vp, _vp]),
         "clsph_nn_infer": (_i32, [_vp, _vp]),
-        "clsph_learned_run": (_i32, [_vp, _vp, _vp, _u32, _u32, ctypes.c_float, _vp, _vp, _vp]),
+        "clsph_learned_run": (_i32, [_vp, _vp, _vp, _u32, _u32, ctypes.c_float, _vp, _vp, _vp,
+                                     _vp]),
         "clsph_nn_profile": (_i32, [_vp, _vp, _u32, _vp, _vp]),
         "clsph_nn_read_graph": (_i32, [_vp, _vp, _vp, _u32, _u32]),
         "clsph_nn_read_output": (_i32, [_vp, _vp, _u32]),
@@ -468,14 +469,17 @@ class Network:
 
     def learned_run(self, learner, judge, macro_steps, judge_every=8, tol=0.01):
         """Judged learned run (clsph_learned_run); returns (judged,
-        fallbacks, [error of each judged step])."""
+        fallbacks, [max error of each judged step], [mean error of each])."""
         nj = (macro_steps + judge_every - 1) // judge_every if judge_every else 0
         errs = np.zeros(max(nj, 1), dtype=np.float32)
+        means = np.zeros(max(nj, 1), dtype=np.float32)
         counts = np.zeros(2, dtype=np.uint32)
         self.ctx.check(lib.clsph_learned_run(
             learner.h, judge.h if judge is not None else None, self.h, macro_steps,
-            judge_every, ctypes.c_float(tol), _ptr(counts[0:1]), _ptr(counts[1:2]), _ptr(errs)))
-        return int(counts[0]), int(counts[1]), [float(e) for e in errs[:int(counts[0])]]
+            judge_every, ctypes.c_float(tol), _ptr(counts[0:1]), _ptr(counts[1:2]), _ptr(errs),
+            _ptr(means)))
+        k = int(counts[0])
+        return (k, int(counts[1]), [float(e) for e in errs[:k]], [float(e) for e in means[:k]])
 
     def compare_density(self, a, b):
         """(max, mean) of |rho_a - rho_b| / rho0 over the same particles."""
