@@ -1,5 +1,6 @@
 // C ABI of the learned force surrogate and the state helpers its training
 // and judging use (include/clsph_hip.h, "learned surrogate" section).
+#include <math.h>
 #include <string.h>
 
 #include <atomic>
@@ -467,6 +468,12 @@ int clsph_learned_run(clsph_state* learner, clsph_state* judge, clsph_nn* m,
                       !learner->ids))
     return fail(m->ctx, CLSPH_EINVAL,
                 "a judged run needs a second state of enough capacity and identities in both");
+  if (judge_every) {
+    const float dl = learner->host_dps[0].p.time_delta, dj = judge->host_dps[0].p.time_delta;
+    if (!(fabsf(dl - (float)nn::kMacro * dj) <= 1e-5f * dl))
+      return fail(m->ctx, CLSPH_EINVAL,
+                  "the learner's time_delta must be 4 times the judge's (one macro step)");
+  }
   uint32_t nj = 0, nf = 0;
   const float rho0 = learner->host_dps[0].p.fluid_density;
   for (uint32_t k = 0; k < macro_steps; ++k) {

@@ -339,3 +339,32 @@ def test_mean_acceleration_target(ctx):
     assert np.abs(want).max() > 1e-3 * np.abs(a0).max()
     # the teacher ended where the twin did
     assert np.array_equal(teacher.download()["position"], twin.download()["position"])
+
+
+def test_judged_run_checks_its_states(ctx):
+    """The main run needs identities in both states and a judge at a
+    quarter of the learner's time step; it runs without a judge when
+    judge_every is 0."""
+    sc = scenes.dam_break(2048, seed=4)
+    dt = float(sc.params["time_delta"])
+    states = {}
+    for name, step in (("learner", 4 * dt), ("judge", dt), ("wrong", 2 * dt)):
+        s = native.State(ctx, sc.n)
+        p = sc.params.copy()
+        p["time_delta"] = np.float32(step)
+        s.set_params(p, sc.terms)
+        s.scene(sc.normals, sc.vertices, sc.indices)
+        s.upload(sc.particles)
+        states[name] = s
+    net = native.Network(ctx, sc.n)
+    net.set_params(np.zeros(net.count, np.float32))
+    with pytest.raises(native.ClsphError):  # no identities yet
+        net.learned_run(states["learner"], states["judge"], 2, 1)
+    for s in states.values():
+        s.track_ids(True)
+    with pytest.raises(native.ClsphError):
+        net.learned_run(states["learner"], states["wrong"], 2, 1)
+    judged, fallbacks, errs, means = net.learned_run(states["learner"], states["judge"], 2, 1)
+    assert judged == 2 and len(errs) == 2 and len(means) == 2
+    assert all(0.0 <= m <= e for m, e in zip(means, errs))
+    assert net.learned_run(states["learner"], None, 3, 0)[:2] == (0, 0)
