@@ -34,6 +34,9 @@ int main(int argc, char** argv) {
   }
 
   int i = 0;
+  // the callbacks only write files: the frame downloads can overlap the
+  // following substeps
+  simulation.callbacks_read_only = true;
   simulation.pre_frame = [&](particle* particles, const simulation_parameters& params,
                              bool full_frame) {
     if (simulation.write_intermediate_frames != full_frame) saver.writeFrameToFile(particles, params);

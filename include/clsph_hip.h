@@ -175,6 +175,22 @@ int clsph_time_stage(clsph_state* st, int stage, uint32_t reps, float* ms);
  * clsph_time_stage; stage 2 includes the state gather into sorted order). */
 int clsph_profile_steps(clsph_state* st, uint32_t nsteps, float* stage_ms);
 
+/* ---- asynchronous frame downloads ----
+ * The reference reads the particle buffer back after every substep
+ * (sph_simulation.cpp:339-341) synchronously. These copy the particles, in
+ * the layout and order of clsph_download_aos, into page-locked host memory
+ * on a second stream: the kernels of the following substeps run while the
+ * copy is in flight and while the host works on an earlier copy. Two slots
+ * (0 and 1) double-buffer the device staging. Single-scene states; the host
+ * array must stay valid until clsph_download_wait returns. */
+int clsph_host_alloc(clsph_ctx* ctx, size_t bytes, void** out);
+int clsph_host_free(clsph_ctx* ctx, void* p);
+int clsph_download_begin(clsph_state* st, uint32_t slot, clsph_particle* out, uint32_t n);
+/* Waits for the download of `slot` and writes the simulation_parameters of
+ * that substep (as clsph_get_grid) to *grid (optional); returns the device
+ * error of that substep, if any. */
+int clsph_download_wait(clsph_state* st, uint32_t slot, clsph_simulation_parameters* grid);
+
 /* ---- learned force surrogate ----
  * No counterpart in the reference, whose only acceleration model is the
  * `forces` kernel (kernels/sph.cl:42-62). A one-round message-passing

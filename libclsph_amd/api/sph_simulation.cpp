@@ -176,6 +176,11 @@ void sph_simulation::simulate(int frame_count) {
   // host_dirty: a callback has had the array since the last upload and may
   // have changed it; the reference re-uploads the host array before every
   // substep (sph_simulation.cpp:195-197), so such changes must reach the device.
+  if (callbacks_read_only && (pre_frame || post_frame)) {
+    simulate_overlapped(frame_count, particles);
+    return;
+  }
+
   bool host_current = true;
   bool host_dirty = false;
   auto sync_host = [&]() {
@@ -216,4 +221,62 @@ void sph_simulation::simulate(int frame_count) {
     }
   }
   CLSPH_CHECK(clsph_sync(ctx), ctx);
+}
+
+// simulate() with read-only callbacks. With S_k the state after k substeps
+// and m substeps per frame, the reference's calls (sph_simulation.cpp:384-401)
+// see, in order: for each k, post_frame(false) and, at a frame end,
+// post_frame(true) on S_k (k >= 1), then pre_frame(true) at a frame start
+// and pre_frame(false) on S_k (k < total). S_0 is the initial host array;
+// each later S_k is downloaded into one of two page-locked buffers on a copy
+// stream, and its callbacks run while the device computes S_k+1.
+void sph_simulation::simulate_overlapped(int frame_count, particle* initial) {
+  clsph_ctx* ctx = impl_->ctx;
+  clsph_state* st = impl_->st;
+  const unsigned int n = parameters.particles_count;
+  int m = 0;
+  while ((float)m < (1.f / parameters.simulation_scale)) ++m;
+  const long long total = (long long)frame_count * m;
+  particle* buf[2] = {nullptr, nullptr};
+  for (int b = 0; b < 2; ++b)
+    CLSPH_CHECK(clsph_host_alloc(ctx, sizeof(particle) * n, reinterpret_cast<void**>(&buf[b])),
+                ctx);
+  auto callbacks = [&](particle* p, long long k) {
+    if (k >= 1 && post_frame) {
+      post_frame(p, parameters, false);
+      if (k % m == 0) post_frame(p, parameters, true);
+    }
+    if (k < total && pre_frame) {
+      if (k % m == 0) pre_frame(p, parameters, true);
+      pre_frame(p, parameters, false);
+    }
+  };
+  auto receive = [&](int slot) {
+    simulation_parameters dev;
+    CLSPH_CHECK(clsph_download_wait(st, (uint32_t)slot, &dev), ctx);
+    parameters.min_point = dev.min_point;
+    parameters.max_point = dev.max_point;
+    parameters.grid_size_x = dev.grid_size_x;
+    parameters.grid_size_y = dev.grid_size_y;
+    parameters.grid_size_z = dev.grid_size_z;
+    parameters.grid_cell_count = dev.grid_cell_count;
+  };
+  callbacks(initial, 0);
+  int pending = -1, slot = 0;
+  for (long long k = 0; k < total; ++k) {
+    CLSPH_CHECK(clsph_step(st, 1), ctx);
+    CLSPH_CHECK(clsph_download_begin(st, (uint32_t)slot, buf[slot], n), ctx);
+    if (pending >= 0) {
+      receive(pending);
+      callbacks(buf[pending], k);
+    }
+    pending = slot;
+    slot ^= 1;
+  }
+  if (pending >= 0) {
+    receive(pending);
+    callbacks(buf[pending], total);
+  }
+  CLSPH_CHECK(clsph_sync(ctx), ctx);
+  for (int b = 0; b < 2; ++b) clsph_host_free(ctx, buf[b]);
 }

@@ -47,9 +47,16 @@ class sph_simulation {
   unsigned int max_bounces = 64;
   // Checkpoint file read at the start of simulate() (sph_simulation.cpp:57-67).
   std::string checkpoint_file = "last_frame.bin";
+  // Set when the callbacks only read the particles (frame writers, probes):
+  // simulate() then downloads each substep asynchronously into one of two
+  // page-locked buffers and runs the callbacks on a substep while the device
+  // computes the next one. The callbacks see the same particles and
+  // parameters, in the same order; changes they make are not uploaded.
+  bool callbacks_read_only = false;
 
  private:
   void init_particles(particle* buffer, const simulation_parameters&);
+  void simulate_overlapped(int frame_count, particle* initial);
   clsph_sim_impl* impl_;
 };
 

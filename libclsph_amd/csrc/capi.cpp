@@ -44,6 +44,20 @@ void free_state_buffers(clsph_state* st) {
   st->graph = nullptr;
   if (st->lgraph) (void)hipGraphExecDestroy(st->lgraph);
   st->lgraph = nullptr;
+  if (st->copy_stream) {
+    (void)hipStreamSynchronize(st->copy_stream);
+    (void)hipStreamDestroy(st->copy_stream);
+    st->copy_stream = nullptr;
+  }
+  for (int k = 0; k < 2; ++k) {
+    dfree(st->stage_aos[k]), dfree(st->stage_meta[k]);
+    if (st->host_meta[k]) (void)hipHostFree(st->host_meta[k]);
+    st->host_meta[k] = nullptr;
+    if (st->ev_pack[k]) (void)hipEventDestroy(st->ev_pack[k]);
+    if (st->ev_done[k]) (void)hipEventDestroy(st->ev_done[k]);
+    st->ev_pack[k] = st->ev_done[k] = nullptr;
+    st->slot_used[k] = false;
+  }
 }
 
 // Work-group tables for every block size, and each scene's range in the
@@ -774,6 +788,83 @@ int clsph_batch_get_grid(clsph_state* st, uint32_t scene, clsph_simulation_param
 
 int clsph_get_grid(clsph_state* st, clsph_simulation_parameters* params) {
   return clsph_batch_get_grid(st, 0, params);
+}
+
+int clsph_host_alloc(clsph_ctx* ctx, size_t bytes, void** out) {
+  if (!ctx || !out || bytes == 0) return fail(ctx, CLSPH_EINVAL, "clsph_host_alloc: bad argument");
+  *out = nullptr;
+  HIPCK(ctx, hipSetDevice(ctx->device));
+  HIPCK(ctx, hipHostMalloc(out, bytes, hipHostMallocDefault));
+  return CLSPH_OK;
+}
+
+int clsph_host_free(clsph_ctx* ctx, void* p) {
+  if (!p) return CLSPH_OK;
+  HIPCK(ctx, hipHostFree(p));
+  return CLSPH_OK;
+}
+
+static constexpr size_t kMetaBytes = sizeof(DevParams) + sizeof(BatchParams);
+
+int clsph_download_begin(clsph_state* st, uint32_t slot, clsph_particle* out, uint32_t n) {
+  if (!st || !out || slot > 1) return fail(st ? st->ctx : nullptr, CLSPH_EINVAL,
+                                          "clsph_download_begin: bad argument");
+  if (st->n_scenes != 1 || n != st->n || n == 0)
+    return fail(st->ctx, CLSPH_EINVAL,
+                "asynchronous downloads take the whole particle count of a single-scene state");
+  clsph_ctx* c = st->ctx;
+  if (!st->copy_stream) {
+    // a second stream at the default priority, so that copies overlap kernels
+    HIPCK(c, hipStreamCreateWithFlags(&st->copy_stream, hipStreamNonBlocking));
+    for (int k = 0; k < 2; ++k) {
+      HIPCK(c, hipEventCreateWithFlags(&st->ev_pack[k], hipEventDisableTiming));
+      HIPCK(c, hipEventCreateWithFlags(&st->ev_done[k], hipEventDisableTiming));
+    }
+  }
+  if (!st->stage_aos[slot]) {
+    int rc = dalloc(c, &st->stage_aos[slot], st->n_cap);
+    if (!rc) rc = dalloc(c, &st->stage_meta[slot], kMetaBytes);
+    if (rc) return rc;
+    HIPCK(c, hipHostMalloc((void**)&st->host_meta[slot], kMetaBytes, hipHostMallocDefault));
+  }
+  hipStream_t s = c->stream;
+  // the staging buffer is free once the copy of its previous download ended
+  if (st->slot_used[slot]) HIPCK(c, hipStreamWaitEvent(s, st->ev_done[slot], 0));
+  launch_pack_aos(st, st->stage_aos[slot]);
+  int rc = check_launch(c);
+  if (rc) return rc;
+  HIPCK(c, hipMemcpyAsync(st->stage_meta[slot], st->dps, sizeof(DevParams),
+                          hipMemcpyDeviceToDevice, s));
+  HIPCK(c, hipMemcpyAsync(st->stage_meta[slot] + sizeof(DevParams), st->bp, sizeof(BatchParams),
+                          hipMemcpyDeviceToDevice, s));
+  HIPCK(c, hipEventRecord(st->ev_pack[slot], s));
+  HIPCK(c, hipStreamWaitEvent(st->copy_stream, st->ev_pack[slot], 0));
+  HIPCK(c, hipMemcpyAsync(out, st->stage_aos[slot], sizeof(clsph_particle) * (size_t)n,
+                          hipMemcpyDeviceToHost, st->copy_stream));
+  HIPCK(c, hipMemcpyAsync(st->host_meta[slot], st->stage_meta[slot], kMetaBytes,
+                          hipMemcpyDeviceToHost, st->copy_stream));
+  HIPCK(c, hipEventRecord(st->ev_done[slot], st->copy_stream));
+  st->slot_used[slot] = true;
+  return CLSPH_OK;
+}
+
+int clsph_download_wait(clsph_state* st, uint32_t slot, clsph_simulation_parameters* grid) {
+  if (!st || slot > 1) return fail(st ? st->ctx : nullptr, CLSPH_EINVAL, "clsph_download_wait: bad argument");
+  if (!st->slot_used[slot])
+    return fail(st->ctx, CLSPH_EINVAL, "no asynchronous download was started in this slot");
+  HIPCK(st->ctx, hipEventSynchronize(st->ev_done[slot]));
+  DevParams d;
+  BatchParams b;
+  memcpy(&d, st->host_meta[slot], sizeof(DevParams));
+  memcpy(&b, st->host_meta[slot] + sizeof(DevParams), sizeof(BatchParams));
+  if (grid) *grid = d.p;
+  const uint32_t err = b.err | d.err;
+  if (err & kErrGridDim)
+    return fail(st->ctx, CLSPH_EGRID,
+                "grid dimension >= 1024 cells on an axis (particles spread too far)");
+  if (err & (kErrTableCap | kErrFineKey))
+    return fail(st->ctx, CLSPH_ETABLE, "grid cells exceed the cell table capacity");
+  return CLSPH_OK;
 }
 
 int clsph_read_order(clsph_state* st, uint32_t* slot, uint32_t* perm, uint32_t n) {

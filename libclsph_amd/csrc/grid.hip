@@ -298,3 +298,51 @@ void launch_track_ids(clsph_state* s) {
   hipLaunchKernelGGL(k_copy_u32, g, dim3(256), 0, s->ctx->stream, s->ids_tmp, s->ids, s->n);
 }
 }  // namespace clsph
+
+namespace clsph {
+
+// Particle records in the reference's layout (80 B: position, velocity,
+// intermediate velocity, acceleration as float4 with w = 0, then density,
+// pressure, grid_index and padding) in canonical order, as the host download
+// assembles them: positions and velocities are canonical already; density,
+// pressure, cell id and, while the fields are canonical, the acceleration
+// are in sorted order and go to canonical slot[k]. Single-scene states.
+__global__ __launch_bounds__(256) void k_pack_aos(
+    const float4* __restrict__ pos, const float4* __restrict__ vel,
+    const float4* __restrict__ vh, const float4* __restrict__ acc_s,
+    const float* __restrict__ rho_s, const float* __restrict__ prs_s,
+    const uint32_t* __restrict__ key, const uint32_t* __restrict__ slot,
+    const DevParams* __restrict__ dps, uint32_t n, int reorder, int want_acc,
+    uint4* __restrict__ out) {
+  uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  auto bits = [](float a, float b, float c) {
+    return make_uint4(__float_as_uint(a), __float_as_uint(b), __float_as_uint(c), 0u);
+  };
+  const float4 p = pos[k], v = vel[k], h = vh[k];
+  uint4* r = out + 5 * (size_t)k;
+  r[0] = bits(p.x, p.y, p.z);
+  r[1] = bits(v.x, v.y, v.z);
+  r[2] = bits(h.x, h.y, h.z);
+  const uint32_t o = reorder ? slot[k] : k;
+  if (o >= n) return;
+  uint4* q = out + 5 * (size_t)o;
+  uint4 a = make_uint4(0u, 0u, 0u, 0u);
+  if (want_acc && !reorder) {
+    const float4 ak = acc_s[k];
+    a = bits(ak.x, ak.y, ak.z);
+  }
+  q[3] = a;
+  const uint32_t gi = reorder ? (key[k] - dps[0].key_base) >> 3 : key[k];
+  q[4] = make_uint4(__float_as_uint(rho_s[k]), __float_as_uint(prs_s[k]), gi, 0u);
+}
+
+void launch_pack_aos(clsph_state* s, clsph_particle* out) {
+  if (s->n == 0) return;
+  hipLaunchKernelGGL(k_pack_aos, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream, s->pos,
+                     s->vel, s->vh, s->acc_s, s->rho_s, s->prs_s, s->key_b, s->slot, s->dps, s->n,
+                     s->fields_canonical ? 0 : 1, s->acc_zero ? 0 : 1,
+                     reinterpret_cast<uint4*>(out));
+}
+
+}  // namespace clsph

@@ -111,6 +111,19 @@ struct clsph_state {
   uint64_t lgraph_serial = 0;
   uint32_t lgraph_n = 0;
 
+  // Asynchronous downloads (clsph_download_begin / clsph_download_wait):
+  // per slot, a device staging buffer of particle records, a device and a
+  // pinned host snapshot of the scene's DevParams and the BatchParams, and
+  // the events ordering pack -> copy -> host; one copy stream. Allocated on
+  // first use.
+  clsph_particle* stage_aos[2] = {};
+  uint8_t* stage_meta[2] = {};
+  uint8_t* host_meta[2] = {};
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t ev_pack[2] = {};
+  hipEvent_t ev_done[2] = {};
+  bool slot_used[2] = {};
+
   // Last stage run (-1 after an upload) and whether the canonical
   // acceleration is the zero vector advection leaves behind.
   int stage = -1;
@@ -136,6 +149,9 @@ void launch_hash(clsph_state* s);
 void launch_permute(clsph_state* s);
 void launch_cell_table(clsph_state* s);
 void launch_track_ids(clsph_state* s);
+// Particle records (reference layout, canonical order) of a single-scene
+// state into out (device, n records), as the host download assembles them.
+void launch_pack_aos(clsph_state* s, clsph_particle* out);
 // sort.hip
 void launch_sort(clsph_state* s);
 uint32_t sort_tiles_for(uint32_t n);

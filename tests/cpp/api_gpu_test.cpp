@@ -128,5 +128,34 @@ int main(int argc, char** argv) {
     EXPECT(got.size() == n && std::memcmp(got.data(), ck.data(), sizeof(particle) * n) == 0);
   }
   std::cout << "ok resume" << std::endl;
+
+  // ---- read-only callbacks: the overlapped path delivers the same particles
+  // and parameters, in the same order, as the synchronous one
+  {
+    std::remove((dir + "/last_frame.bin").c_str());
+    std::vector<std::vector<particle>> seen[2];
+    std::vector<std::pair<int, unsigned>> meta[2];
+    for (int mode = 0; mode < 2; ++mode) {
+      sph_simulation s;
+      setup(s, n, dir);
+      s.callbacks_read_only = mode == 1;
+      s.pre_frame = [&](particle* p, const simulation_parameters& prm, bool full) {
+        seen[mode].emplace_back(p, p + prm.particles_count);
+        meta[mode].emplace_back(full ? 2 : 3, prm.grid_cell_count);
+      };
+      s.post_frame = [&](particle* p, const simulation_parameters& prm, bool full) {
+        seen[mode].emplace_back(p, p + prm.particles_count);
+        meta[mode].emplace_back(full ? 1 : 0, prm.grid_cell_count);
+      };
+      s.simulate();
+    }
+    EXPECT(seen[0].size() == 30 && seen[1].size() == seen[0].size());
+    EXPECT(meta[0] == meta[1]);
+    bool same = seen[0].size() == seen[1].size();
+    for (size_t k = 0; same && k < seen[0].size(); ++k)
+      same = std::memcmp(seen[0][k].data(), seen[1][k].data(), sizeof(particle) * n) == 0;
+    EXPECT(same);
+  }
+  std::cout << "ok overlapped_callbacks" << std::endl;
   return fails ? 1 : 0;
 }

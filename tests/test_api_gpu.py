@@ -22,7 +22,7 @@ def test_simulate_callbacks_and_resume(tmp_path):
     r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
-    for name in ("callbacks", "abi_agreement", "resume"):
+    for name in ("callbacks", "abi_agreement", "resume", "overlapped_callbacks"):
         assert f"ok {name}" in r.stdout
 
 
