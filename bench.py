@@ -171,6 +171,7 @@ def main():
     # Per-stage durations over the same substeps: the timed region replayed
     # stage by stage from its starting state, HIP events between stages.
     prof = start.profile_steps(args.steps if args.profile_steps <= 0 else args.profile_steps)
+    pairs = start.neighbour_pairs()  # found by the last replayed substep
     start.close()
     grid = st.grid()
     cells = int(grid["grid_cell_count"])
@@ -213,6 +214,13 @@ def main():
         "data": "synthetic",
         "config": cfg,
         "stages_ms": {k: v["ms"] for k, v in stages.items()},
+        # density and forces are gather work: their pair-interaction rates,
+        # with the neighbour pairs (r < h, ordered) of the last timed substep
+        "pair_interactions": {
+            "pairs_per_substep": pairs,
+            "density_pairs_per_s": pairs / (prof["density"] * 1e-3),
+            "forces_pairs_per_s": pairs / (prof["forces"] * 1e-3),
+        },
         "roofline": {
             "kernel": dom,
             "bound": "hbm",

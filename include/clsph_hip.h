@@ -175,6 +175,11 @@ int clsph_time_stage(clsph_state* st, int stage, uint32_t reps, float* ms);
  * clsph_time_stage; stage 2 includes the state gather into sorted order). */
 int clsph_profile_steps(clsph_state* st, uint32_t nsteps, float* stage_ms);
 
+/* Number of ordered neighbour pairs (r < h, self excluded) the last density
+ * stage found, summed over all particles: the pair interactions of density
+ * and forces. A diagnostic with no counterpart in the reference. */
+int clsph_neighbour_pairs(clsph_state* st, uint64_t* pairs);
+
 /* ---- asynchronous frame downloads ----
  * The reference reads the particle buffer back after every substep
  * (sph_simulation.cpp:339-341) synchronously. These copy the particles, in

@@ -790,6 +790,19 @@ int clsph_get_grid(clsph_state* st, clsph_simulation_parameters* params) {
   return clsph_batch_get_grid(st, 0, params);
 }
 
+int clsph_neighbour_pairs(clsph_state* st, uint64_t* pairs) {
+  if (!st || !pairs) return fail(st ? st->ctx : nullptr, CLSPH_EINVAL, "NULL argument");
+  if (st->stage < kStageDensity && st->stage != kStageAdvect)
+    return fail(st->ctx, CLSPH_EINVAL, "no density stage has run on this state");
+  std::vector<uint32_t> cnt(st->n);
+  int rc = read_u32(st, st->nbr_count, cnt.data(), st->n);
+  if (rc) return rc;
+  uint64_t e = 0;
+  for (uint32_t c : cnt) e += c;
+  *pairs = e;
+  return CLSPH_OK;
+}
+
 int clsph_host_alloc(clsph_ctx* ctx, size_t bytes, void** out) {
   if (!ctx || !out || bytes == 0) return fail(ctx, CLSPH_EINVAL, "clsph_host_alloc: bad argument");
   *out = nullptr;

@@ -93,6 +93,7 @@ def _load():
         "clsph_nn_get_params": (_i32, [_vp, _vp, _u32]),
         "clsph_nn_forward": (_i32, [_vp, _vp, _vp]),
         "clsph_nn_infer": (_i32, [_vp, _vp]),
+        "clsph_neighbour_pairs": (_i32, [_vp, _vp]),
         "clsph_learned_run": (_i32, [_vp, _vp, _vp, _u32, _u32, ctypes.c_float, _vp, _vp, _vp,
                                      _vp]),
         "clsph_nn_profile": (_i32, [_vp, _vp, _u32, _vp, _vp]),
@@ -327,6 +328,12 @@ class State:
         ms = np.zeros(7, dtype=np.float32)
         self.ctx.check(lib.clsph_profile_steps(self.h, nsteps, _ptr(ms)))
         return {name: float(ms[i]) for name, i in STAGES.items()}
+
+    def neighbour_pairs(self):
+        """Ordered neighbour pairs (r < h) found by the last density stage."""
+        e = np.zeros(1, dtype=np.uint64)
+        self.ctx.check(lib.clsph_neighbour_pairs(self.h, _ptr(e)))
+        return int(e[0])
 
     def time_stage(self, name, reps):
         ms = np.zeros(1, dtype=np.float32)
