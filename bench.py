@@ -109,6 +109,10 @@ def main():
                     help="substeps of the per-stage timing (0: the timed substeps)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--direct", action="store_true",
+                    help="launch every substep stage by stage instead of replaying its captured "
+                         "graph, timing the stages on the way (for kernel tracing; implies "
+                         "--no-learned)")
     ap.add_argument("--track-ids", action="store_true",
                     help="also track particle identities each substep; its copy kernel "
                          "(k_copy_u32, 4 n bytes read and written) calibrates HBM counters")
@@ -151,28 +155,42 @@ def main():
     if args.track_ids:
         st.track_ids(True)
 
-    if args.warmup:
-        st.step(args.warmup)
-    # the state at the start of the timed region, to time its stages below
-    start = native.State(ctx, n)
-    start.set_params(scene.params, scene.terms)
-    start.scene(scene.normals, scene.vertices, scene.indices)
-    start.copy_from(st)
-    ctx.sync()
-    t0 = time.perf_counter()
-    ms = st.time_steps(args.steps)  # HIP events around K graph replays
-    ctx.sync()
-    wall = time.perf_counter() - t0
+    if args.direct:
+        # every substep launched stage by stage, HIP events between stages
+        if args.warmup:
+            st.profile_steps(args.warmup)
+        ctx.sync()
+        t0 = time.perf_counter()
+        prof = st.profile_steps(args.steps)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        ms = wall * 1e3
+        pairs = st.neighbour_pairs()
+    else:
+        if args.warmup:
+            st.step(args.warmup)
+        # the state at the start of the timed region, to time its stages below
+        start = native.State(ctx, n)
+        start.set_params(scene.params, scene.terms)
+        start.scene(scene.normals, scene.vertices, scene.indices)
+        start.copy_from(st)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ms = st.time_steps(args.steps)  # HIP events around K graph replays
+        ctx.sync()
+        wall = time.perf_counter() - t0
     ms_per_step = ms / args.steps
     value = n * args.steps / (ms / 1e3)
     print(f"# timed {args.steps} substeps: {ms:.2f} ms (events), {wall * 1e3:.2f} ms (wall)",
           file=sys.stderr, flush=True)
 
-    # Per-stage durations over the same substeps: the timed region replayed
-    # stage by stage from its starting state, HIP events between stages.
-    prof = start.profile_steps(args.steps if args.profile_steps <= 0 else args.profile_steps)
-    pairs = start.neighbour_pairs()  # found by the last replayed substep
-    start.close()
+    if not args.direct:
+        # Per-stage durations over the same substeps: the timed region
+        # replayed stage by stage from its starting state, HIP events between
+        # stages.
+        prof = start.profile_steps(args.steps if args.profile_steps <= 0 else args.profile_steps)
+        pairs = start.neighbour_pairs()  # found by the last replayed substep
+        start.close()
     grid = st.grid()
     cells = int(grid["grid_cell_count"])
     nbytes = stage_bytes(n, cells)
@@ -188,13 +206,14 @@ def main():
     capped = st.bounce_capped()
 
     learned = None
-    if args.workload == "dam_break" and not args.no_learned:
+    if args.workload == "dam_break" and not (args.no_learned or args.direct):
         learned = bench_learned(ctx, st, scene, ms_per_step, args.learned_macro_steps,
                                 judge_every=args.judge_every)
 
     cfg.update({
         "dt": float(scene.params["time_delta"]),
         "timed_substeps": f"{args.warmup + 1}-{args.warmup + args.steps}",
+        "launch": "stage by stage" if args.direct else "graph replay",
         "grid_cell_count_end": cells,
         "state_finite": finite,
         "bounce_capped_last": capped,

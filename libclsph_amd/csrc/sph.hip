@@ -127,42 +127,45 @@ __device__ inline void for_each_run(const uint32_t hc[3], const float fr[3], uin
   }
 }
 
-// Besides density and pressure, records the particle's neighbours (r < h,
-// self excluded) in visiting order, four to a uint4, into
-// nbr[q * n + i] (q < kMaxNeighbours / 4), the count into nbr_count[i], and
-// the force inputs fa[i] = (x, p / rho^2) and fb[i] = (v, m / rho).
-// Candidates are loaded four at a time so that their latencies overlap.
-__global__ __launch_bounds__(kMaxBlock) void k_density(const float4* __restrict__ pos,
-                                                       const float4* __restrict__ vel,
-                                                       const uint32_t* __restrict__ table,
-                                                       const BlockDesc* __restrict__ blocks,
-                                                       const DevParams* __restrict__ dps,
-                                                       const BatchParams* __restrict__ bp,
-                                                       float* __restrict__ rho,
-                                                       float* __restrict__ prs,
-                                                       float4* __restrict__ fa,
-                                                       float4* __restrict__ fb,
-                                                       uint4* __restrict__ nbr,
-                                                       uint32_t* __restrict__ nbr_count,
-                                                       uint32_t n) {
-  uint32_t scene;
-  uint32_t i = particle_of(blocks, dps, scene);
-  if (i == 0xFFFFFFFFu) return;
-  const DevParams* dp = dps + scene;
-  const float h = dp->p.h, h2 = h * h;
-  const uint32_t F = usable_key_end(dp, bp);
-  float4 xi = pos[i];
-  uint32_t hc[3];
-  float fr[3];
-  half_cell_frac(xi, dp, hc, fr);
+// Density sum and neighbour list of one particle i. The list holds the
+// neighbours (r < h, self excluded) in visiting order, four to a uint4, at
+// nbr[q * n + i] (q < kMaxNeighbours / 4); pending entries wait in a shift
+// register whose newest entry is p3.
+struct DensityWalk {
   float acc = 0.f;
-  uint32_t cnt = 0;
-  // pending list entries, a shift register: the newest is p3
-  uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+  uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0, cnt = 0;
+
+  // Candidate j at squared distance d2; in_run is false for the padding
+  // entries of a group of four.
+  __device__ inline void visit(bool in_run, float d2, float h2, uint32_t j, uint32_t i,
+                               uint4* __restrict__ nbr, uint32_t n) {
+    if (in_run && d2 < h2) {
+      float t = h2 - d2;
+      acc += t * t * t;
+      const bool app = j != i;
+      const bool keep = app && cnt < kMaxNeighbours;
+      p0 = keep ? p1 : p0;
+      p1 = keep ? p2 : p1;
+      p2 = keep ? p3 : p2;
+      p3 = keep ? j : p3;
+      if (keep && (cnt & 3u) == 3u) nbr[(size_t)(cnt >> 2) * n + i] = make_uint4(p0, p1, p2, p3);
+      cnt += app ? 1u : 0u;
+    }
+  }
+};
+
+// The walk over the sorted positions in HBM: the reach-pruned half-cell runs
+// of for_each_run, candidates loaded four at a time so that their latencies
+// overlap (pos is padded by 4 entries past the particles, so a group needs no
+// clamp).
+__device__ inline void density_walk_global(DensityWalk& w, uint32_t i, float4 xi,
+                                           const uint32_t hc[3], const float fr[3],
+                                           const DevParams* __restrict__ dp, uint32_t F,
+                                           const float4* __restrict__ pos,
+                                           const uint32_t* __restrict__ table, float h2,
+                                           uint4* __restrict__ nbr, uint32_t n) {
   for_each_run(hc, fr, dp->key_base, table, F, n, [&](uint32_t s, uint32_t e) {
     for (uint32_t j0 = s; j0 < e; j0 += 4) {
-      // pos is padded by 4 entries past the particles, so the group's
-      // loads need no clamp; entries at or past e are masked out below
       const float4* pj = pos + j0;
       float4 c[4];
 #pragma unroll
@@ -171,33 +174,212 @@ __global__ __launch_bounds__(kMaxBlock) void k_density(const float4* __restrict_
       for (int u = 0; u < 4; ++u) {
         const uint32_t j = j0 + u;
         float dx = xi.x - c[u].x, dy = xi.y - c[u].y, dz = xi.z - c[u].z;
-        float d2 = dx * dx + dy * dy + dz * dz;
-        if (j < e && d2 < h2) {
-          float t = h2 - d2;
-          acc += t * t * t;
-          const bool app = j != i;
-          const bool keep = app && cnt < kMaxNeighbours;
-          p0 = keep ? p1 : p0;
-          p1 = keep ? p2 : p1;
-          p2 = keep ? p3 : p2;
-          p3 = keep ? j : p3;
-          if (keep && (cnt & 3u) == 3u) nbr[(size_t)(cnt >> 2) * n + i] = make_uint4(p0, p1, p2, p3);
-          cnt += app ? 1u : 0u;
-        }
+        w.visit(j < e, dx * dx + dy * dy + dz * dz, h2, j, i, nbr, n);
       }
     }
   });
-  const uint32_t r = cnt & 3u;
+}
+
+// The same walk over a cell's halo staged in LDS: the 4x4x4 half-cells
+// around the particle's cell (side 2h), half-cell q = x + 4y + 16z at
+// cand[off[q], off[q + 1]) in sorted order, each entry (x, y, z, sorted
+// index). The particle sits at local coordinates 1 + (hc & 1), so each row
+// (dy, dz) of its 3x3x3 block is one contiguous range. Visiting order, and
+// so every sum and list, is that of density_walk_global.
+__device__ inline void density_walk_halo(DensityWalk& w, uint32_t i, float4 xi,
+                                         const uint32_t hc[3], const float fr[3],
+                                         const float4* cand, const uint32_t* off, float h2,
+                                         uint4* __restrict__ nbr, uint32_t n) {
+  const uint32_t lx = 1u + (hc[0] & 1u), ly = 1u + (hc[1] & 1u), lz = 1u + (hc[2] & 1u);
+  const float lo_x = fr[0] * fr[0], hi_x = (1.f - fr[0]) * (1.f - fr[0]);
+  for (int dz = -1; dz <= 1; ++dz) {
+    const float az = dz < 0 ? fr[2] : (dz > 0 ? 1.f - fr[2] : 0.f);
+    for (int dy = -1; dy <= 1; ++dy) {
+      const float ay = dy < 0 ? fr[1] : (dy > 0 ? 1.f - fr[1] : 0.f);
+      const float ryz = az * az + ay * ay;
+      if (ryz >= kReach2) continue;
+      const uint32_t row = 4u * (ly + (uint32_t)dy) + 16u * (lz + (uint32_t)dz);
+      const uint32_t xa = lo_x + ryz < kReach2 ? lx - 1u : lx;
+      const uint32_t xb = hi_x + ryz < kReach2 ? lx + 1u : lx;
+      const uint32_t s = off[row + xa], e = off[row + xb + 1u];
+      for (uint32_t j0 = s; j0 < e; j0 += 4) {
+        float4 c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) c[u] = cand[j0 + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          float dx = xi.x - c[u].x, dy2 = xi.y - c[u].y, dz2 = xi.z - c[u].z;
+          w.visit(j0 + u < e, dx * dx + dy2 * dy2 + dz2 * dz2, h2, __float_as_uint(c[u].w), i,
+                  nbr, n);
+        }
+      }
+    }
+  }
+}
+
+// Inverse of spread10: the 10 bits at positions 3k.
+__device__ inline uint32_t compact10(uint32_t v) {
+  v &= 0x09249249u;
+  v = (v | (v >> 2)) & 0x030C30C3u;
+  v = (v | (v >> 4)) & 0x0300F00Fu;
+  v = (v | (v >> 8)) & 0x030000FFu;
+  v = (v | (v >> 16)) & 0x000003FFu;
+  return v;
+}
+
+// Halo staging of the density kernel, per work-group of WAVES waves: up to
+// 2 WAVES cells (their 4x4x4 half-cell halos) and 768 WAVES candidates
+// (x, y, z, index: 16 B each). A bulk cell holds about 64 particles, so the
+// work-group's particles span about WAVES + 1 cells of about 512 halo
+// candidates each.
+template <int WAVES>
+struct DensityHalos {
+  static constexpr uint32_t kHalos = 2u * WAVES;
+  static constexpr uint32_t kCap = 768u * WAVES - 4u;
+  float4 cand[kCap + 4];               // + the padding of a group of four
+  uint32_t off[kHalos * 64u + 1u];     // halo h, half-cell q: off[64 h + q]
+  uint32_t cell[kHalos];               // first key of each cell (a multiple of 8)
+  uint32_t wave_sum[WAVES];
+  uint32_t wave_cells[WAVES];
+};
+
+// Density and pressure (sph.cl:9-40, forces.cl:15-43) of the particles
+// [first, first + 64 WAVES) of one scene, with each particle's neighbour
+// list (self excluded) in visiting order into nbr[q * n + i], its count into
+// nbr_count[i] and the force inputs fa[i] = (x, p / rho^2), fb[i] = (v, m /
+// rho). The work-group first stages, for each distinct cell (side 2h) its
+// particles fall in, the cell's 4x4x4 half-cell halo in LDS (coalesced runs
+// of the sorted positions); particles then walk their 3x3x3 half-cells
+// there. Cells beyond the staging capacity, and every particle when the
+// table is unusable, walk the sorted positions in HBM instead, with the same
+// visiting order.
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void k_density(const float4* __restrict__ pos,
+                                                        const float4* __restrict__ vel,
+                                                        const uint32_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ table,
+                                                        const BlockDesc* __restrict__ blocks,
+                                                        const DevParams* __restrict__ dps,
+                                                        const BatchParams* __restrict__ bp,
+                                                        float* __restrict__ rho,
+                                                        float* __restrict__ prs,
+                                                        float4* __restrict__ fa,
+                                                        float4* __restrict__ fb,
+                                                        uint4* __restrict__ nbr,
+                                                        uint32_t* __restrict__ nbr_count,
+                                                        uint32_t n) {
+  using Halos = DensityHalos<WAVES>;
+  __shared__ Halos sh;
+  const BlockDesc b = blocks[blockIdx.x];
+  const DevParams* dp = dps + b.scene;
+  const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+  const uint32_t se = dp->begin + dp->n;
+  const uint32_t i = b.first + t;
+  const bool live = i < se;
+  const float h = dp->p.h, h2 = h * h;
+  const uint32_t F = usable_key_end(dp, bp);
+
+  // Cells of the work-group's particles, in order: a particle starts one
+  // when it is the first particle or its cell differs from its predecessor's.
+  const uint32_t key = live ? keys[i] : 0u;
+  const uint32_t prev = (live && t > 0) ? keys[i - 1] : 0u;
+  const bool starts = live && (t == 0 || (key >> 3) != (prev >> 3));
+  const uint64_t mask = __ballot(starts);
+  if (lane == 0) sh.wave_cells[wv] = (uint32_t)__popcll(mask);
+  __syncthreads();
+  uint32_t before = 0, cells = 0;
+  for (int q = 0; q < WAVES; ++q) {
+    const uint32_t c = sh.wave_cells[q];
+    before += (uint32_t)q < wv ? c : 0u;
+    cells += c;
+  }
+  // index of the particle's cell among the work-group's cells
+  const uint32_t ci = before + (uint32_t)__popcll(mask & (~0ull >> (63u - lane))) - 1u;
+  if (starts && ci < Halos::kHalos) sh.cell[ci] = key & ~7u;
+  const uint32_t halos = F == 0u ? 0u : (cells < Halos::kHalos ? cells : Halos::kHalos);
+  __syncthreads();
+
+  // Halo half-cell runs: thread t takes the entries 2t and 2t + 1 of the
+  // (halo, half-cell) list.
+  uint32_t rs[2] = {0u, 0u}, rc[2] = {0u, 0u};
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const uint32_t q = 2u * t + (uint32_t)u, hh = q >> 6, l = q & 63u;
+    if (hh < halos) {
+      const uint32_t c8 = sh.cell[hh];
+      const uint32_t cl = (c8 - dp->key_base) >> 3;
+      const uint32_t hx = 2u * compact10(cl) - 1u + (l & 3u);
+      const uint32_t hy = 2u * compact10(cl >> 1) - 1u + ((l >> 2) & 3u);
+      const uint32_t hz = 2u * compact10(cl >> 2) - 1u + (l >> 4);
+      if (hx < 2048u && hy < 2048u && hz < 2048u) {
+        const uint32_t f = dp->key_base + half_key(hx, hy, hz);
+        if (f < F) {
+          uint32_t e = table[f + 1u];
+          e = e < n ? e : n;
+          uint32_t s = table[f];
+          s = s < e ? s : e;
+          rs[u] = s;
+          rc[u] = e - s;
+        }
+      }
+    }
+  }
+  // exclusive scan of the run lengths over the work-group
+  const uint32_t mine = rc[0] + rc[1];
+  uint32_t incl = mine;
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) sh.wave_sum[wv] = incl;
+  __syncthreads();
+  uint32_t base = 0;
+  for (int q = 0; q < WAVES; ++q) base += (uint32_t)q < wv ? sh.wave_sum[q] : 0u;
+  const uint32_t ex = base + incl - mine;
+  sh.off[2u * t] = ex;
+  sh.off[2u * t + 1u] = ex + rc[0];
+  if (t == WAVES * 64 - 1) sh.off[Halos::kHalos * 64u] = ex + mine;
+  __syncthreads();
+  // halos staged: the leading ones whose candidates fit
+  uint32_t staged = 0;
+  while (staged < halos && sh.off[64u * (staged + 1u)] <= Halos::kCap) ++staged;
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const uint32_t q = 2u * t + (uint32_t)u;
+    if ((q >> 6) < staged) {
+      const uint32_t o = u ? ex + rc[0] : ex;
+      for (uint32_t k = 0; k < rc[u]; ++k) {
+        const uint32_t j = rs[u] + k;
+        const float4 p = pos[j];
+        sh.cand[o + k] = make_float4(p.x, p.y, p.z, __uint_as_float(j));
+      }
+    }
+  }
+  __syncthreads();
+
+  if (!live) return;
+  const float4 xi = pos[i];
+  uint32_t hc[3];
+  float fr[3];
+  half_cell_frac(xi, dp, hc, fr);
+  DensityWalk w;
+  if (ci < staged)
+    density_walk_halo(w, i, xi, hc, fr, sh.cand, sh.off + 64u * ci, h2, nbr, n);
+  else
+    density_walk_global(w, i, xi, hc, fr, dp, F, pos, table, h2, nbr, n);
+  const uint32_t cnt = w.cnt, r = cnt & 3u;
   if (cnt < kMaxNeighbours && r) {
     // the r pending entries are p(4-r) .. p3, oldest first
-    uint4 last = r == 1 ? make_uint4(p3, 0u, 0u, 0u)
-                        : (r == 2 ? make_uint4(p2, p3, 0u, 0u) : make_uint4(p1, p2, p3, 0u));
+    uint4 last = r == 1 ? make_uint4(w.p3, 0u, 0u, 0u)
+                        : (r == 2 ? make_uint4(w.p2, w.p3, 0u, 0u)
+                                  : make_uint4(w.p1, w.p2, w.p3, 0u));
     nbr[(size_t)(cnt >> 2) * n + i] = last;
   }
   nbr_count[i] = cnt;
   // density = sum m * poly6 * (h^2 - r^2)^3 ; Tait pressure (sph.cl:36-39)
   float m = dp->p.particle_mass;
-  float d = m * (dp->t.poly_6 * acc);
+  float d = m * (dp->t.poly_6 * w.acc);
   float q = d / dp->p.fluid_density;
   float q2 = q * q, q4 = q2 * q2;
   float p = dp->p.K * (q4 * q2 * q - 1.f);
@@ -344,10 +526,18 @@ __global__ __launch_bounds__(kMaxBlock) void k_forces(const float4* __restrict__
 
 void launch_density(clsph_state* s) {
   if (s->n == 0) return;
-  hipLaunchKernelGGL(k_density, dim3(s->nblocks_of(kStageDensity)), dim3(s->bs_of(kStageDensity)),
-                     0, s->ctx->stream, s->pos_s, s->vel_s, s->table,
-                     s->table_of(kStageDensity), s->dps, s->bp, s->rho_s, s->prs_s, s->fa, s->fb,
-                     s->nbr, s->nbr_count, s->n);
+  const int bs = s->bs_of(kStageDensity);
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(s->nblocks_of(kStageDensity)), dim3(bs), 0, s->ctx->stream,
+                       s->pos_s, s->vel_s, s->key_b, s->table, s->table_of(kStageDensity), s->dps,
+                       s->bp, s->rho_s, s->prs_s, s->fa, s->fb, s->nbr, s->nbr_count, s->n);
+  };
+  switch (bs) {
+    case 64: go(k_density<1>); break;
+    case 128: go(k_density<2>); break;
+    case 256: go(k_density<4>); break;
+    default: go(k_density<8>); break;
+  }
 }
 
 void launch_forces(clsph_state* s) {

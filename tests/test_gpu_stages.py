@@ -176,3 +176,47 @@ def test_upload_download_roundtrip(ctx):
             np.testing.assert_array_equal(back[f][:, :3], parts[f][:, :3], err_msg=f)
         else:
             np.testing.assert_array_equal(back[f], parts[f], err_msg=f)
+
+
+def density_forces_with_block_size(ctx, sc, bs):
+    st = native.State(ctx, sc.n, table_cap=1 << 22)
+    st.set_params(sc.params, sc.terms)
+    st.scene(sc.normals, sc.vertices, sc.indices)
+    st.set_block_sizes([bs] * 7)
+    st.upload(sc.particles)
+    for name in ("grid", "hash", "sort", "cell_table", "density"):
+        st.stage(name)
+    dens = st.download_sorted()
+    pairs = st.neighbour_pairs()
+    st.stage("forces")
+    forc = st.download_sorted()
+    st.close()
+    return dens, forc, pairs
+
+
+# dam4096 at 64 threads per work-group stages fewer halos than fit at 512, so
+# its particles take both walks; dense4000 puts 4000 particles in one cell,
+# beyond any work-group's staging capacity, so every particle walks HBM.
+@pytest.mark.parametrize("case", ["dam4096", "cloud4097", "dense4000"])
+def test_density_walks_agree_bit_for_bit(ctx, case):
+    """The density kernel stages the halo of each cell its work-group's
+    particles fall in (as many as fit) in LDS and walks HBM for the rest, in
+    the same visiting order: densities, pressures, neighbour lists and hence
+    accelerations do not depend on the work-group size, which changes how
+    many halos are staged."""
+    print(f"case {case}", flush=True)
+    sc = (cloud(4000, 8, lo=0.49, hi=0.51) if case == "dense4000" else CASES[case]())
+    runs = {bs: density_forces_with_block_size(ctx, sc, bs) for bs in (64, 128, 256, 512)}
+    d0, f0, p0 = runs[256]
+    for bs, (d, f, p) in runs.items():
+        np.testing.assert_array_equal(d["density"], d0["density"], err_msg=f"density bs={bs}")
+        np.testing.assert_array_equal(d["pressure"], d0["pressure"], err_msg=f"pressure bs={bs}")
+        np.testing.assert_array_equal(f["acceleration"], f0["acceleration"],
+                                      err_msg=f"acceleration bs={bs}")
+        assert p == p0
+    if case == "dense4000":
+        ref = oracle.stages(sc.particles, sc.params, sc.terms, sc.normals, sc.vertices,
+                            sc.indices)
+        rd = ref.density["density"]
+        # 4000-term FP32 sums, added in different orders by the two
+        assert np.max(np.abs(d0["density"] - rd) / np.abs(rd)) <= 1e-4
