@@ -22,7 +22,7 @@ import numpy as np
 CANDIDATES = (64, 128, 256, 512)
 # Stages that run over the work-group table (stage order of clsph_time_stage).
 TUNED = {"hash": 1, "density": 4, "forces": 5, "advect": 6}
-DEFAULT = [256] * 7
+DEFAULT = [256, 256, 256, 256, 512, 256, 256]  # the state's own defaults
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "mi355x.json")
 
 
@@ -42,7 +42,8 @@ def load_table(path=TABLE):
 
 
 def sizes_for(counts, table=None):
-    """Work-group sizes for a batch with these scene counts (default 256)."""
+    """Work-group sizes for a batch with these scene counts (the defaults
+    without a table)."""
     table = load_table() if table is None else table
     if not table:
         return list(DEFAULT)

@@ -139,16 +139,20 @@ struct DensityWalk {
   // entries of a group of four.
   __device__ inline void visit(bool in_run, float d2, float h2, uint32_t j, uint32_t i,
                                uint4* __restrict__ nbr, uint32_t n) {
+    static_assert((kMaxNeighbours & (kMaxNeighbours - 1u)) == 0u && kMaxNeighbours >= 4u,
+                  "the list-room test below needs a power of two");
     if (in_run && d2 < h2) {
       float t = h2 - d2;
       acc += t * t * t;
       const bool app = j != i;
-      const bool keep = app && cnt < kMaxNeighbours;
-      p0 = keep ? p1 : p0;
-      p1 = keep ? p2 : p1;
-      p2 = keep ? p3 : p2;
-      p3 = keep ? j : p3;
-      if (keep && (cnt & 3u) == 3u) nbr[(size_t)(cnt >> 2) * n + i] = make_uint4(p0, p1, p2, p3);
+      // the register shifts on every neighbour and is stored after each
+      // fourth one while the list has room: cnt % 4 == 3 and cnt < kMaxNeighbours
+      p0 = app ? p1 : p0;
+      p1 = app ? p2 : p1;
+      p2 = app ? p3 : p2;
+      p3 = app ? j : p3;
+      if (app && (cnt & (3u - kMaxNeighbours)) == 3u)
+        nbr[(size_t)(cnt >> 2) * n + i] = make_uint4(p0, p1, p2, p3);
       cnt += app ? 1u : 0u;
     }
   }
@@ -206,11 +210,12 @@ __device__ inline void density_walk_halo(DensityWalk& w, uint32_t i, float4 xi,
         float4 c[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) c[u] = cand[j0 + u];
+        const uint32_t rem = e - j0;  // entries of the group inside the range (>= 1)
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           float dx = xi.x - c[u].x, dy2 = xi.y - c[u].y, dz2 = xi.z - c[u].z;
-          w.visit(j0 + u < e, dx * dx + dy2 * dy2 + dz2 * dz2, h2, __float_as_uint(c[u].w), i,
-                  nbr, n);
+          w.visit((uint32_t)u < rem, dx * dx + dy2 * dy2 + dz2 * dz2, h2,
+                  __float_as_uint(c[u].w), i, nbr, n);
         }
       }
     }
@@ -228,14 +233,16 @@ __device__ inline uint32_t compact10(uint32_t v) {
 }
 
 // Halo staging of the density kernel, per work-group of WAVES waves: up to
-// 2 WAVES cells (their 4x4x4 half-cell halos) and 768 WAVES candidates
-// (x, y, z, index: 16 B each). A bulk cell holds about 64 particles, so the
+// kHalos cells (their 4x4x4 half-cell halos) and kCap candidates (x, y, z,
+// index: 16 B each). A bulk cell holds about 64 particles, so the
 // work-group's particles span about WAVES + 1 cells of about 512 halo
-// candidates each.
+// candidates each. Up to 4 waves, 768 candidates per wave (three 4-wave
+// work-groups per CU); at 8 waves, 12 cells and 4860 candidates keep a
+// work-group within 80 KB, so that two fit per CU.
 template <int WAVES>
 struct DensityHalos {
-  static constexpr uint32_t kHalos = 2u * WAVES;
-  static constexpr uint32_t kCap = 768u * WAVES - 4u;
+  static constexpr uint32_t kHalos = WAVES < 8 ? 2u * WAVES : 12u;
+  static constexpr uint32_t kCap = WAVES < 8 ? 768u * WAVES - 4u : 4860u;
   float4 cand[kCap + 4];               // + the padding of a group of four
   uint32_t off[kHalos * 64u + 1u];     // halo h, half-cell q: off[64 h + q]
   uint32_t cell[kHalos];               // first key of each cell (a multiple of 8)
@@ -269,6 +276,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_density(const float4* __restrict
                                                         uint32_t* __restrict__ nbr_count,
                                                         uint32_t n) {
   using Halos = DensityHalos<WAVES>;
+  static_assert(Halos::kHalos <= 2u * WAVES, "at most two (halo, half-cell) entries per thread");
   __shared__ Halos sh;
   const BlockDesc b = blocks[blockIdx.x];
   const DevParams* dp = dps + b.scene;
@@ -337,8 +345,10 @@ __global__ __launch_bounds__(WAVES * 64) void k_density(const float4* __restrict
   uint32_t base = 0;
   for (int q = 0; q < WAVES; ++q) base += (uint32_t)q < wv ? sh.wave_sum[q] : 0u;
   const uint32_t ex = base + incl - mine;
-  sh.off[2u * t] = ex;
-  sh.off[2u * t + 1u] = ex + rc[0];
+  if (2u * t < Halos::kHalos * 64u) {  // threads past the last halo hold no runs
+    sh.off[2u * t] = ex;
+    sh.off[2u * t + 1u] = ex + rc[0];
+  }
   if (t == WAVES * 64 - 1) sh.off[Halos::kHalos * 64u] = ex + mine;
   __syncthreads();
   // halos staged: the leading ones whose candidates fit

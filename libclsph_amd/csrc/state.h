@@ -82,7 +82,9 @@ struct clsph_state {
   // each stage runs with. Advection and the bounding-box kernel share one.
   clsph::BlockDesc* blocks[clsph::kNumBlockSizes] = {};
   uint32_t nblocks[clsph::kNumBlockSizes] = {};
-  int stage_bs[kNumStages] = {2, 2, 2, 2, 2, 2, 2};  // indices into kBlockSizes
+  // indices into kBlockSizes: 256 threads, 512 for density (two 80 KB
+  // halo-staging work-groups per CU, measured fastest at 1M particles)
+  int stage_bs[kNumStages] = {2, 2, 2, 2, 3, 2, 2};
 
   // Collision meshes of all scenes, concatenated (uploaded once;
   // sph_simulation.cpp:296-319 re-uploads it every substep).
