@@ -30,7 +30,7 @@ int read_dps(clsph_state* st, std::vector<DevParams>& out, BatchParams* bp = nul
 void free_state_buffers(clsph_state* st) {
   dfree(st->pos), dfree(st->vel), dfree(st->vh);
   dfree(st->pos_s), dfree(st->vel_s), dfree(st->vh_s), dfree(st->acc_s);
-  dfree(st->rho_s), dfree(st->prs_s), dfree(st->fa), dfree(st->fb);
+  dfree(st->rho_s), dfree(st->prs_s), dfree(st->fab);
   dfree(st->nbr), dfree(st->nbr_count);
   dfree(st->key_a), dfree(st->val_a), dfree(st->key_b), dfree(st->val_b);
   dfree(st->hist), dfree(st->digit_base), dfree(st->table), dfree(st->slot);
@@ -337,7 +337,7 @@ int alloc_state(clsph_ctx* ctx, uint32_t n_scenes, const uint32_t* caps, bool fi
   st->n_tiles_cap = sort_tiles_for(n_cap ? n_cap : 1);
   int rc = CLSPH_OK;
   float4** f4[] = {&st->pos, &st->vel, &st->vh, &st->vel_s, &st->vh_s,
-                   &st->acc_s, &st->fa, &st->fb};
+                   &st->acc_s};
   for (auto p : f4)
     if (rc == CLSPH_OK) rc = dalloc(ctx, p, n_cap);
   // sorted positions carry 4 padding entries: density loads candidates in
@@ -345,6 +345,7 @@ int alloc_state(clsph_ctx* ctx, uint32_t n_scenes, const uint32_t* caps, bool fi
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->pos_s, (size_t)n_cap + 4);
   if (rc == CLSPH_OK)
     HIPCK(ctx, hipMemset(st->pos_s + n_cap, 0, 4 * sizeof(float4)));
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->fab, 2 * (size_t)n_cap);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->rho_s, n_cap);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->prs_s, n_cap);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->nbr, (size_t)(kMaxNeighbours / 4) * n_cap);

@@ -389,13 +389,14 @@ __global__ void k_node_inputs(const float* __restrict__ rho, const float* __rest
 
 // Edge inputs e (32 bf16 per edge), after k_node_inputs.
 __global__ __launch_bounds__(256) void k_edge_inputs(
-    const float4* __restrict__ fa, const float4* __restrict__ fb, const float2* __restrict__ nu,
+    const float4* __restrict__ fab, const float2* __restrict__ nu,
     const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint32_t e,
     EdgeConsts kc, uint16_t* __restrict__ E1) {
   for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < e; k += gridDim.x * 256) {
     uint32_t i = dst[k], j = src[k];
     float f[kFeat];
-    edge_features(fa[i], fb[i], fa[j], fb[j], nu[i], nu[j], kc, f);
+    edge_features(fab[2u * i], fab[2u * i + 1u], fab[2u * j], fab[2u * j + 1u], nu[i], nu[j],
+                  kc, f);
     u32x4* o = (u32x4*)(E1 + (size_t)k * kFeat);
     for (int q = 0; q < kFeat / 8; ++q) {
       uint32_t packed[4];
@@ -516,8 +517,8 @@ struct EdgeFetch {
 };
 
 __device__ __attribute__((always_inline)) inline EdgeFetch fetch_edge(
-    uint32_t k, uint32_t i, uint32_t n, uint32_t deg, const float4* __restrict__ fa,
-    const float4* __restrict__ fb, const float2* __restrict__ nu,
+    uint32_t k, uint32_t i, uint32_t n, uint32_t deg, const float4* __restrict__ fab,
+    const float2* __restrict__ nu,
     const uint4* __restrict__ nbr) {
   const int g = (threadIdx.x & 63) >> 4;
   EdgeFetch e;
@@ -527,8 +528,8 @@ __device__ __attribute__((always_inline)) inline EdgeFetch fetch_edge(
     uint4 js = nbr[(size_t)k * n + i];
     j = g == 0 ? js.x : g == 1 ? js.y : g == 2 ? js.z : js.w;
   }
-  e.a = fa[j];
-  e.b = fb[j];
+  e.a = fab[2u * j];
+  e.b = fab[2u * j + 1u];
   e.nu = nu[j];
   return e;
 }
@@ -580,7 +581,7 @@ __device__ __attribute__((always_inline)) inline void edge_round(
 // forces kernel wrote for them); otherwise a0 = acc[i].
 template <bool OWN>
 __global__ __launch_bounds__(kSurWaves * 64) void k_surrogate(
-    const float4* __restrict__ fa, const float4* __restrict__ fb, const float2* __restrict__ nu,
+    const float4* __restrict__ fab, const float2* __restrict__ nu,
     const float* __restrict__ rho, const uint4* __restrict__ nbr,
     const uint32_t* __restrict__ count, uint32_t n, EdgeConsts kc, ForceTail ft,
     const uint16_t* __restrict__ wb, const float* __restrict__ params, float4* __restrict__ acc,
@@ -625,7 +626,7 @@ __global__ __launch_bounds__(kSurWaves * 64) void k_surrogate(
       dmax = max(dmax, (uint32_t)__shfl_xor((int)dmax, o, 64));
     }
     const uint32_t cnt = valid ? count[i] : 0u;
-    const float4 ai = fa[i], bi = fb[i];
+    const float4 ai = fab[2u * i], bi = fab[2u * i + 1u];
     const float2 ni = nu[i];
     f32x4 s[2];
     for (int ob = 0; ob < 2; ++ob) s[ob] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -634,10 +635,10 @@ __global__ __launch_bounds__(kSurWaves * 64) void k_surrogate(
     // round's neighbour inputs are fetched one round ahead
     const uint32_t kfull = dmin >> 2, kend = (dmax + 3) >> 2;
     EdgeFetch cur;
-    if (kend) cur = fetch_edge(0, i, n, deg, fa, fb, nu, nbr);
+    if (kend) cur = fetch_edge(0, i, n, deg, fab, nu, nbr);
     for (uint32_t k = 0; k < kend; ++k) {
       EdgeFetch nxt = cur;
-      if (k + 1 < kend) nxt = fetch_edge(k + 1, i, n, deg, fa, fb, nu, nbr);
+      if (k + 1 < kend) nxt = fetch_edge(k + 1, i, n, deg, fab, nu, nbr);
       if (k < kfull)
         edge_round<false, OWN>(k, deg, ai, bi, ni, cur, kc, ft, wlds, w1s, zb, s, ps);
       else
@@ -1015,7 +1016,7 @@ void launch_edge_inputs(clsph_nn* m, clsph_state* st) {
                      m->src, m->dst, m->degc);
   hipLaunchKernelGGL(k_node_inputs, grid_for(n), dim3(256), 0, s, st->rho_s, st->prs_s, n,
                      d.p.fluid_density, d.p.K, (float2*)m->nu);
-  hipLaunchKernelGGL(k_edge_inputs, grid_for(e), dim3(256), 0, s, st->fa, st->fb,
+  hipLaunchKernelGGL(k_edge_inputs, grid_for(e), dim3(256), 0, s, st->fab,
                      (const float2*)m->nu, m->src, m->dst, e, edge_consts(d), m->E1);
 }
 
@@ -1067,12 +1068,12 @@ void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float* out, bool o
   if (blocks * kSurWaves > ntiles) blocks = (ntiles + kSurWaves - 1) / kSurWaves;
   if (blocks == 0) blocks = 1;
   if (own_forces)
-    hipLaunchKernelGGL(k_surrogate<true>, dim3(blocks), dim3(kSurWaves * 64), 0, s, st->fa,
-                       st->fb, (const float2*)m->nu, st->rho_s, st->nbr, st->nbr_count, n,
+    hipLaunchKernelGGL(k_surrogate<true>, dim3(blocks), dim3(kSurWaves * 64), 0, s, st->fab,
+                       (const float2*)m->nu, st->rho_s, st->nbr, st->nbr_count, n,
                        edge_consts(d), force_tail(d), m->wb, m->params, acc, out);
   else
-    hipLaunchKernelGGL(k_surrogate<false>, dim3(blocks), dim3(kSurWaves * 64), 0, s, st->fa,
-                       st->fb, (const float2*)m->nu, st->rho_s, st->nbr, st->nbr_count, n,
+    hipLaunchKernelGGL(k_surrogate<false>, dim3(blocks), dim3(kSurWaves * 64), 0, s, st->fab,
+                       (const float2*)m->nu, st->rho_s, st->nbr, st->nbr_count, n,
                        edge_consts(d), force_tail(d), m->wb, m->params, acc, out);
   if (ev) (void)hipEventRecord(ev[1], s);
 }

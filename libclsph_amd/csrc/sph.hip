@@ -253,8 +253,8 @@ struct DensityHalos {
 // Density and pressure (sph.cl:9-40, forces.cl:15-43) of the particles
 // [first, first + 64 WAVES) of one scene, with each particle's neighbour
 // list (self excluded) in visiting order into nbr[q * n + i], its count into
-// nbr_count[i] and the force inputs fa[i] = (x, p / rho^2), fb[i] = (v, m /
-// rho). The work-group first stages, for each distinct cell (side 2h) its
+// nbr_count[i] and the force inputs fab[2i] = (x, p / rho^2), fab[2i + 1] =
+// (v, m / rho). The work-group first stages, for each distinct cell (side 2h) its
 // particles fall in, the cell's 4x4x4 half-cell halo in LDS (coalesced runs
 // of the sorted positions); particles then walk their 3x3x3 half-cells
 // there. Cells beyond the staging capacity, and every particle when the
@@ -270,8 +270,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_density(const float4* __restrict
                                                         const BatchParams* __restrict__ bp,
                                                         float* __restrict__ rho,
                                                         float* __restrict__ prs,
-                                                        float4* __restrict__ fa,
-                                                        float4* __restrict__ fb,
+                                                        float4* __restrict__ fab,
                                                         uint4* __restrict__ nbr,
                                                         uint32_t* __restrict__ nbr_count,
                                                         uint32_t n) {
@@ -396,8 +395,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_density(const float4* __restrict
   rho[i] = d;
   prs[i] = p;
   float4 vi = vel[i];
-  fa[i] = make_float4(xi.x, xi.y, xi.z, p / (d * d));
-  fb[i] = make_float4(vi.x, vi.y, vi.z, m / d);
+  fab[2u * i] = make_float4(xi.x, xi.y, xi.z, p / (d * d));
+  fab[2u * i + 1u] = make_float4(vi.x, vi.y, vi.z, m / d);
 }
 
 // Pair terms of compute_internal_forces_with_grid (forces.cl:60-101) for one
@@ -443,8 +442,7 @@ __device__ inline void force_pair(ForceSums& f, const ForceConsts& k, float4 ai,
 // (more than kMaxNeighbours), for callers that sum the listed ones
 // themselves.
 template <bool OVERFLOW_ONLY>
-__global__ __launch_bounds__(kMaxBlock) void k_forces(const float4* __restrict__ fa,
-                                                      const float4* __restrict__ fb,
+__global__ __launch_bounds__(kMaxBlock) void k_forces(const float4* __restrict__ fab,
                                                       const float* __restrict__ rho,
                                                       const uint32_t* __restrict__ table,
                                                       const uint4* __restrict__ nbr,
@@ -470,7 +468,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_forces(const float4* __restrict__
   const float h3 = k.h2 * k.h;
   k.spiky0 = -45.f / (3.14159274f * (h3 * h3));
   const uint32_t F = usable_key_end(dp, bp);
-  float4 ai = fa[i], bi = fb[i];
+  float4 ai = fab[2u * i], bi = fab[2u * i + 1u];
   float rho_i = rho[i];
   ForceSums fs;
   fs.pt = make_float3(0.f, 0.f, 0.f);
@@ -488,8 +486,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_forces(const float4* __restrict__
     for (uint32_t gq = 0; gq < groups; ++gq) {
       uint4 jn = js;
       if (gq + 1 < groups) jn = nbr[(size_t)(gq + 1) * n + i];
-      float4 a0 = fa[js.x], b0 = fb[js.x], a1 = fa[js.y], b1 = fb[js.y];
-      float4 a2 = fa[js.z], b2 = fb[js.z], a3 = fa[js.w], b3 = fb[js.w];
+      float4 a0 = fab[2u * js.x], b0 = fab[2u * js.x + 1u];
+      float4 a1 = fab[2u * js.y], b1 = fab[2u * js.y + 1u];
+      float4 a2 = fab[2u * js.z], b2 = fab[2u * js.z + 1u];
+      float4 a3 = fab[2u * js.w], b3 = fab[2u * js.w + 1u];
       force_pair(fs, k, ai, bi, a0, b0);
       force_pair(fs, k, ai, bi, a1, b1);
       force_pair(fs, k, ai, bi, a2, b2);
@@ -500,9 +500,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_forces(const float4* __restrict__
     if (q < cnt) {
       uint4 js = nbr[(size_t)(q >> 2) * n + i];
       uint32_t r = cnt - q;
-      force_pair(fs, k, ai, bi, fa[js.x], fb[js.x]);
-      if (r > 1) force_pair(fs, k, ai, bi, fa[js.y], fb[js.y]);
-      if (r > 2) force_pair(fs, k, ai, bi, fa[js.z], fb[js.z]);
+      force_pair(fs, k, ai, bi, fab[2u * js.x], fab[2u * js.x + 1u]);
+      if (r > 1) force_pair(fs, k, ai, bi, fab[2u * js.y], fab[2u * js.y + 1u]);
+      if (r > 2) force_pair(fs, k, ai, bi, fab[2u * js.z], fab[2u * js.z + 1u]);
     }
   } else {
     // More neighbours than the list holds: walk the half-cells again.
@@ -510,11 +510,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_forces(const float4* __restrict__
     half_cell(ai, dp, hc);
     const float h2 = k.h2;
     for_each_candidate(hc, dp->key_base, table, F, n, [&](uint32_t j) {
-      float4 aj = fa[j];
+      float4 aj = fab[2u * j];
       float dx = ai.x - aj.x, dy = ai.y - aj.y, dz = ai.z - aj.z;
       float d2 = dx * dx + dy * dy + dz * dz;
       if (!(d2 < h2) || j == i) return;
-      force_pair(fs, k, ai, bi, aj, fb[j]);
+      force_pair(fs, k, ai, bi, aj, fab[2u * j + 1u]);
     });
   }
   float3 pt = fs.pt, vt = fs.vt, nrm = fs.nrm;
@@ -540,7 +540,7 @@ void launch_density(clsph_state* s) {
   auto go = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, dim3(s->nblocks_of(kStageDensity)), dim3(bs), 0, s->ctx->stream,
                        s->pos_s, s->vel_s, s->key_b, s->table, s->table_of(kStageDensity), s->dps,
-                       s->bp, s->rho_s, s->prs_s, s->fa, s->fb, s->nbr, s->nbr_count, s->n);
+                       s->bp, s->rho_s, s->prs_s, s->fab, s->nbr, s->nbr_count, s->n);
   };
   switch (bs) {
     case 64: go(k_density<1>); break;
@@ -553,7 +553,7 @@ void launch_density(clsph_state* s) {
 void launch_forces(clsph_state* s) {
   if (s->n == 0) return;
   hipLaunchKernelGGL((k_forces<false>), dim3(s->nblocks_of(kStageForces)),
-                     dim3(s->bs_of(kStageForces)), 0, s->ctx->stream, s->fa, s->fb, s->rho_s,
+                     dim3(s->bs_of(kStageForces)), 0, s->ctx->stream, s->fab, s->rho_s,
                      s->table, s->nbr, s->nbr_count, s->table_of(kStageForces), s->dps, s->bp,
                      s->acc_s, s->n);
 }
@@ -561,7 +561,7 @@ void launch_forces(clsph_state* s) {
 void launch_forces_overflow(clsph_state* s) {
   if (s->n == 0) return;
   hipLaunchKernelGGL((k_forces<true>), dim3(s->nblocks_of(kStageForces)),
-                     dim3(s->bs_of(kStageForces)), 0, s->ctx->stream, s->fa, s->fb, s->rho_s,
+                     dim3(s->bs_of(kStageForces)), 0, s->ctx->stream, s->fab, s->rho_s,
                      s->table, s->nbr, s->nbr_count, s->table_of(kStageForces), s->dps, s->bp,
                      s->acc_s, s->n);
 }

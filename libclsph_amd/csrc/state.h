@@ -46,8 +46,9 @@ struct clsph_state {
   float4* acc_s = nullptr;  // acceleration from forces
   float* rho_s = nullptr;   // density
   float* prs_s = nullptr;   // pressure
-  float4* fa = nullptr;     // force inputs: (x, y, z, p / rho^2)
-  float4* fb = nullptr;     // force inputs: (vx, vy, vz, m / rho)
+  // force inputs, interleaved so that a neighbour's two records share a
+  // cache line: fab[2i] = (x, y, z, p / rho^2), fab[2i + 1] = (vx, vy, vz, m / rho)
+  float4* fab = nullptr;
   uint4* nbr = nullptr;           // neighbour lists, 4 per entry: nbr[q * n + i]
   uint32_t* nbr_count = nullptr;  // neighbours per particle
 
