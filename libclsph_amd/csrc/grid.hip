@@ -191,20 +191,76 @@ __global__ __launch_bounds__(256) void k_cell_table(const uint32_t* __restrict__
     }
     uint32_t end = lower_bound(keys, lo, hi, nxt);
     ht[k0] = start;
-    uint32_t b = start;
-    for (uint32_t o = 1; o < 8; ++o) {
-      b = lower_bound(keys, b, end, k0 | o);
-      ht[k0 + o] = b;
+    // the octants' bounds inside [start, end): seven searches stepping
+    // together, so that each step issues up to seven independent loads
+    uint32_t ol[7], oh[7];
+#pragma unroll
+    for (int o = 0; o < 7; ++o) ol[o] = start, oh[o] = end;
+    bool more = start < end;
+    while (more) {
+      more = false;
+#pragma unroll
+      for (int o = 0; o < 7; ++o) {
+        if (ol[o] < oh[o]) {
+          const uint32_t mid = (ol[o] + oh[o]) >> 1;
+          if (keys[mid] < (k0 | (uint32_t)(o + 1)))
+            ol[o] = mid + 1;
+          else
+            oh[o] = mid;
+          more = more || ol[o] < oh[o];
+        }
+      }
     }
+#pragma unroll
+    for (int o = 0; o < 7; ++o) ht[k0 + 1 + o] = ol[o];
     if (c == cells - 1) ht[total] = n;
   }
 }
 
-// Slot of each sorted particle in the reference's order (stable by cell id).
-// Within a cell the half-cell order is octant-major with the canonical
-// (input) order kept inside each octant, so the particle's rank in its cell
-// is its index in its octant plus the members of the other octants that came
-// before it in the input.
+// Slot of the particle at sorted position k in the reference's order (stable
+// by cell id). Within a cell the half-cell order is octant-major with the
+// canonical (input) order kept inside each octant, so the particle's rank in
+// its cell is its index in its octant plus, for each other octant, the
+// members that came before it in the input: a lower bound of its canonical
+// index in that octant's run. The seven searches step together, so each
+// step issues up to seven independent loads.
+__device__ inline uint32_t slot_by_search(uint32_t k, const uint32_t* __restrict__ keys,
+                                          const uint32_t* __restrict__ perm,
+                                          const uint32_t* __restrict__ ht, uint32_t total) {
+  const uint32_t f = keys[k], c8 = f & ~7u, o = f & 7u;
+  if (f >= total || c8 + 8u > total) return k;  // keys that are not this table's
+  const uint32_t me = perm[k];
+  uint32_t b[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) b[q] = ht[c8 + q];
+  uint32_t lo[8], hi[8], base = 0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    lo[q] = b[q];
+    hi[q] = (uint32_t)q == o ? b[q] : b[q + 1];  // the own octant is counted directly
+    base += b[q];
+  }
+  bool more = true;
+  while (more) {
+    more = false;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (lo[q] < hi[q]) {
+        const uint32_t mid = (lo[q] + hi[q]) >> 1;
+        if (perm[mid] < me)
+          lo[q] = mid + 1;
+        else
+          hi[q] = mid;
+        more = more || lo[q] < hi[q];
+      }
+    }
+  }
+  uint32_t rank = k - b[o];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) rank += lo[q];
+  return b[0] + rank - base;
+}
+
 __global__ __launch_bounds__(256) void k_canonical_slot(const uint32_t* __restrict__ keys,
                                                         const uint32_t* __restrict__ perm,
                                                         const uint32_t* __restrict__ ht,
@@ -212,27 +268,12 @@ __global__ __launch_bounds__(256) void k_canonical_slot(const uint32_t* __restri
                                                         uint32_t* __restrict__ slot, uint32_t n) {
   uint32_t k = blockIdx.x * 256 + threadIdx.x;
   if (k >= n) return;
-  if (bp->key_total > bp->table_cap || (bp->err & kErrFineKey)) {
+  const uint32_t total = bp->key_total;
+  if (total > bp->table_cap || (bp->err & kErrFineKey)) {
     slot[k] = k;
     return;
   }
-  uint32_t f = keys[k], c8 = f & ~7u, o = f & 7u;
-  uint32_t me = perm[k];
-  uint32_t rank = k - ht[f];
-  for (uint32_t q = 0; q < 8; ++q) {
-    if (q == o) continue;
-    uint32_t lo = ht[c8 + q], hi = ht[c8 + q + 1];
-    uint32_t a = lo, b = hi;
-    while (a < b) {
-      uint32_t mid = (a + b) >> 1;
-      if (perm[mid] < me)
-        a = mid + 1;
-      else
-        b = mid;
-    }
-    rank += a - lo;
-  }
-  slot[k] = ht[c8] + rank;
+  slot[k] = slot_by_search(k, keys, perm, ht, total);
 }
 
 void launch_aabb(clsph_state* s) {

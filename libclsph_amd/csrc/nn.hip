@@ -509,25 +509,31 @@ __device__ __attribute__((always_inline)) inline void silu_operands(const f32x4*
 }
 
 // The neighbour of lane (g, c) in round k: edge 4 k + g of particle c (the
-// particle itself when that edge is past its degree), with its inputs.
+// particle itself when that edge is past its degree).
+__device__ __attribute__((always_inline)) inline uint32_t edge_index(
+    uint32_t k, uint32_t i, uint32_t n, uint32_t deg, const uint4* __restrict__ nbr) {
+  const int g = (threadIdx.x & 63) >> 4;
+  uint32_t j = i;
+  if (4 * k + g < deg) {
+    uint4 js = nbr[(size_t)k * n + i];
+    j = g == 0 ? js.x : g == 1 ? js.y : g == 2 ? js.z : js.w;
+  }
+  return j;
+}
+
 struct EdgeFetch {
   float4 a, b;
   float2 nu;
   bool mine;
 };
 
+// The inputs of neighbour j, lane (g, c)'s edge of round k.
 __device__ __attribute__((always_inline)) inline EdgeFetch fetch_edge(
-    uint32_t k, uint32_t i, uint32_t n, uint32_t deg, const float4* __restrict__ fab,
-    const float2* __restrict__ nu,
-    const uint4* __restrict__ nbr) {
+    uint32_t k, uint32_t j, uint32_t deg, const float4* __restrict__ fab,
+    const float2* __restrict__ nu) {
   const int g = (threadIdx.x & 63) >> 4;
   EdgeFetch e;
   e.mine = 4 * k + g < deg;
-  uint32_t j = i;
-  if (e.mine) {
-    uint4 js = nbr[(size_t)k * n + i];
-    j = g == 0 ? js.x : g == 1 ? js.y : g == 2 ? js.z : js.w;
-  }
   e.a = fab[2u * j];
   e.b = fab[2u * j + 1u];
   e.nu = nu[j];
@@ -634,11 +640,15 @@ __global__ __launch_bounds__(kSurWaves * 64) void k_surrogate(
     // rounds inside every column's degree, then the masked tail; each
     // round's neighbour inputs are fetched one round ahead
     const uint32_t kfull = dmin >> 2, kend = (dmax + 3) >> 2;
+    // neighbour indices are loaded two rounds ahead, so that a round's
+    // gathers never wait for the index load that names them
     EdgeFetch cur;
-    if (kend) cur = fetch_edge(0, i, n, deg, fab, nu, nbr);
+    if (kend) cur = fetch_edge(0, edge_index(0, i, n, deg, nbr), deg, fab, nu);
+    uint32_t jn = kend > 1 ? edge_index(1, i, n, deg, nbr) : i;
     for (uint32_t k = 0; k < kend; ++k) {
       EdgeFetch nxt = cur;
-      if (k + 1 < kend) nxt = fetch_edge(k + 1, i, n, deg, fab, nu, nbr);
+      if (k + 1 < kend) nxt = fetch_edge(k + 1, jn, deg, fab, nu);
+      if (k + 2 < kend) jn = edge_index(k + 2, i, n, deg, nbr);
       if (k < kfull)
         edge_round<false, OWN>(k, deg, ai, bi, ni, cur, kc, ft, wlds, w1s, zb, s, ps);
       else
