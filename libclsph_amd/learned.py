@@ -27,6 +27,7 @@ and when max_i |rho_i - rho_i^plain| / rho0 exceeds `tol` (1 %), the macro
 step is replaced by the plain result (a fallback).
 
     python -m libclsph_amd.learned --out weights.npy --trajectories 200 --minutes 8
+    python -m libclsph_amd.learned --profile     (times the shipped network at 1M)
 """
 import argparse
 import math
@@ -204,8 +205,28 @@ def run_learned(ctx, learner, teacher, net, macro_steps, dt, judge_every=8, tol=
     return r
 
 
+def profile_network(substeps=20, reps=5):
+    """Times the fused network on the 1,048,576-particle dam break after
+    `substeps` plain substeps, every kernel launched directly (no graphs), for
+    kernel counters; returns (ms per evaluation, edges)."""
+    ctx = native.Context(0)
+    sc = scenes.dam_break(1048576)
+    st = native.State(ctx, sc.n)
+    st.set_params(sc.params, sc.terms)
+    st.scene(sc.normals, sc.vertices, sc.indices)
+    st.upload(sc.particles)
+    if substeps:
+        st.profile_steps(substeps)
+    forces_stages(st)
+    net = native.Network(ctx, sc.n, e_cap=1024)
+    net.set_params(np.load(WEIGHTS))
+    return net.profile(st, reps)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--profile", action="store_true",
+                    help="time the shipped network on the 1M dam break instead of training")
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--trajectories", type=int, default=200)
     ap.add_argument("--steps", type=int, default=2000)
@@ -215,6 +236,10 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--init", default=None, help="start from these weights")
     a = ap.parse_args()
+    if a.profile:
+        ms, edges = profile_network()
+        print(f"network {ms:.4f} ms per evaluation, {edges} edges", flush=True)
+        return
     ctx = native.Context(0)
     n_cap = 101 * 202 * 12  # largest slab variant
     net = native.Network(ctx, n_cap, e_cap=80 * n_cap)
