@@ -478,23 +478,34 @@ __global__ __launch_bounds__(kMaxBlock) void k_forces(const float4* __restrict__
   fs.lap = bi.w * k.c_lap * k.h2 * (3.f * k.h2);
   uint32_t cnt = nbr_count[i];
   if (cnt <= kMaxNeighbours) {
-    // groups of four neighbours; the next group's indices load while the
-    // current group's particles are gathered
+    // groups of four neighbours, software-pipelined: while a group's pair
+    // terms are summed, the next group's records and the indices of the one
+    // after it are in flight
     const uint32_t groups = cnt >> 2;
-    uint4 js = make_uint4(0u, 0u, 0u, 0u);
-    if (groups) js = nbr[i];
+    auto gather = [&](uint4 js, float4 ra[4], float4 rb[4]) {
+      const uint32_t jj[4] = {js.x, js.y, js.z, js.w};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        ra[u] = fab[2u * jj[u]];
+        rb[u] = fab[2u * jj[u] + 1u];
+      }
+    };
+    float4 ca[4], cb[4];
+    uint4 jn = make_uint4(0u, 0u, 0u, 0u);
+    if (groups) {
+      gather(nbr[i], ca, cb);
+      if (groups > 1) jn = nbr[(size_t)n + i];
+    }
     for (uint32_t gq = 0; gq < groups; ++gq) {
-      uint4 jn = js;
-      if (gq + 1 < groups) jn = nbr[(size_t)(gq + 1) * n + i];
-      float4 a0 = fab[2u * js.x], b0 = fab[2u * js.x + 1u];
-      float4 a1 = fab[2u * js.y], b1 = fab[2u * js.y + 1u];
-      float4 a2 = fab[2u * js.z], b2 = fab[2u * js.z + 1u];
-      float4 a3 = fab[2u * js.w], b3 = fab[2u * js.w + 1u];
-      force_pair(fs, k, ai, bi, a0, b0);
-      force_pair(fs, k, ai, bi, a1, b1);
-      force_pair(fs, k, ai, bi, a2, b2);
-      force_pair(fs, k, ai, bi, a3, b3);
-      js = jn;
+      float4 na[4], nb[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) na[u] = ca[u], nb[u] = cb[u];
+      if (gq + 1 < groups) gather(jn, na, nb);
+      if (gq + 2 < groups) jn = nbr[(size_t)(gq + 2) * n + i];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) force_pair(fs, k, ai, bi, ca[u], cb[u]);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) ca[u] = na[u], cb[u] = nb[u];
     }
     const uint32_t q = groups << 2;
     if (q < cnt) {
