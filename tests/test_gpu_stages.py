@@ -220,3 +220,21 @@ def test_density_walks_agree_bit_for_bit(ctx, case):
         rd = ref.density["density"]
         # 4000-term FP32 sums, added in different orders by the two
         assert np.max(np.abs(d0["density"] - rd) / np.abs(rd)) <= 1e-4
+
+
+def test_table_overflow_is_reported(ctx):
+    """A grid with more cells than the state's table holds is reported as
+    CLSPH_ETABLE at the next synchronisation. Meanwhile the table kernels
+    skip their work and density finds no candidates (its table lookups are
+    off), so the substep still runs to its end."""
+    print("case table overflow", flush=True)
+    sc = scenes.dam_break(4096, seed=7)
+    st = native.State(ctx, sc.n, table_cap=64)
+    st.set_params(sc.params, sc.terms)
+    st.scene(sc.normals, sc.vertices, sc.indices)
+    st.upload(sc.particles)
+    st.step(1)
+    with pytest.raises(native.ClsphError) as err:
+        st.download()
+    assert err.value.code == -5
+    st.close()
