@@ -258,6 +258,8 @@ __global__ __launch_bounds__(256) void k_graph(const uint4* __restrict__ nbr,
 // Constants of the edge inputs (one scene).
 struct EdgeConsts {
   float h, h2, inv_h, rho0, inv_rho0, K, m, mu_over_m, c_spiky, c_visc, c_grad, c_lap;
+  // the edge inputs' constant factors, multiplied out once (edge_features)
+  float kp, kv, kg, kl, kr, kd, kx;
 };
 
 inline EdgeConsts edge_consts(const DevParams& d) {
@@ -274,6 +276,15 @@ inline EdgeConsts edge_consts(const DevParams& d) {
   k.c_visc = d.t.viscosity;
   k.c_grad = d.t.poly_6_gradient;
   k.c_lap = d.t.poly_6_laplacian;
+  const double inv_a = 1.0 / kAccScale, inv_v = 1.0 / kVelScale, h = d.p.h;
+  const double m = d.p.particle_mass;
+  k.kp = (float)(-m * d.t.spiky * inv_a);
+  k.kv = (float)((double)d.p.dynamic_viscosity / m * d.t.viscosity * inv_a);
+  k.kg = (float)((double)d.t.poly_6_gradient * h);
+  k.kl = (float)((double)d.t.poly_6_laplacian * h * h);
+  k.kr = (float)(-m * d.t.spiky * h * inv_v * inv_a);
+  k.kd = (float)(m * d.t.poly_6_gradient * 1000.0 * h * inv_v / (double)d.p.fluid_density);
+  k.kx = (float)(10.0 * inv_v / h);
   return k;
 }
 
@@ -344,39 +355,42 @@ __device__ __attribute__((always_inline)) inline void pair_terms(PairSums& f, co
 // 10 / (h V).
 __device__ inline void edge_features(float4 ai, float4 bi, float4 aj, float4 bj, float2 ni,
                                      float2 nj, const EdgeConsts& k, float f[kFeat]) {
-  constexpr float inv_a = 1.f / kAccScale, inv_v = 1.f / kVelScale;
+  static_assert(kVelScale == 1.f, "velocity inputs are taken unscaled");
   float dx = ai.x - aj.x, dy = ai.y - aj.y, dz = ai.z - aj.z;
   float d2 = dx * dx + dy * dy + dz * dz;
   float r = __builtin_amdgcn_sqrtf(d2);  // the forces kernel's square root
   float q = r * k.inv_h;
   float w = 1.f - q * q;
+  const float vx = bi.x - bj.x, vy = bi.y - bj.y, vz = bi.z - bj.z;
   f[0] = dx * k.inv_h, f[1] = dy * k.inv_h, f[2] = dz * k.inv_h;
-  f[3] = (bi.x - bj.x) * inv_v, f[4] = (bi.y - bj.y) * inv_v, f[5] = (bi.z - bj.z) * inv_v;
+  f[3] = vx, f[4] = vy, f[5] = vz;
   f[6] = q, f[7] = w * w * w;
   f[8] = ni.x, f[9] = ni.y, f[10] = nj.x, f[11] = nj.y;
-  const float hr = k.h - r, t = k.h2 - d2;
-  const float ps = (ai.w + aj.w) * k.m;
-  const float g = r >= 1e-7f ? ps * k.c_spiky * hr * hr * __builtin_amdgcn_rcpf(r) : 0.f;
-  f[12] = -g * dx * inv_a, f[13] = -g * dy * inv_a, f[14] = -g * dz * inv_a;
-  // mu / rho_i = (m / rho_i) (mu / m)
-  const float vw = bi.w * k.mu_over_m * bj.w * k.c_visc * hr * inv_a;
-  f[15] = (bj.x - bi.x) * vw, f[16] = (bj.y - bi.y) * vw, f[17] = (bj.z - bi.z) * vw;
-  const float gw = bj.w * k.c_grad * t * t * k.h;
+  const float hr = k.h - r, t = k.h2 - d2, t2 = t * t;
+  const float ws = ai.w + aj.w;  // p_i / rho_i^2 + p_j / rho_j^2
+  const bool apart = r >= 1e-7f;
+  const float ir = __builtin_amdgcn_rcpf(r);
+  // pressure pair acceleration / A: -(ws m c_spiky (h - r)^2 / r) x / A
+  const float g = apart ? ws * k.kp * hr * hr * ir : 0.f;
+  f[12] = g * dx, f[13] = g * dy, f[14] = g * dz;
+  // viscosity pair acceleration / A: (v_j - v_i) (m / rho_j) (mu / rho_i) c_visc (h - r) / A
+  const float vw = bi.w * k.kv * bj.w * hr;
+  f[15] = -vx * vw, f[16] = -vy * vw, f[17] = -vz * vw;
+  // colour-field gradient term times h, laplacian term times h^2
+  const float gw = bj.w * k.kg * t2;
   f[18] = gw * dx, f[19] = gw * dy, f[20] = gw * dz;
-  f[21] = bj.w * k.c_lap * t * (3.f * k.h2 - 7.f * d2) * k.h2;
-  const float vx = bi.x - bj.x, vy = bi.y - bj.y, vz = bi.z - bj.z;
+  f[21] = bj.w * k.kl * t * (3.f * k.h2 - 7.f * d2);
   const float xv = dx * vx + dy * vy + dz * vz;
   float ra = 0.f, rb = 0.f;
-  if (r >= 1e-7f) {
-    const float ir = __builtin_amdgcn_rcpf(r);
+  if (apart) {
     ra = hr * hr * ir;
-    rb = xv * hr * (k.h + r) * ir * ir * ir;
+    rb = xv * hr * (k.h + r) * (ir * ir * ir);
   }
-  const float cr = -ps * k.c_spiky * k.h * inv_v * inv_a;
+  const float cr = ws * k.kr;
   f[22] = cr * (ra * vx - rb * dx), f[23] = cr * (ra * vy - rb * dy);
   f[24] = cr * (ra * vz - rb * dz);
-  f[25] = k.m * k.c_grad * t * t * xv * (1000.f * k.h * inv_v) * k.inv_rho0;
-  f[26] = xv * (10.f * inv_v) * k.inv_h;
+  f[25] = t2 * xv * k.kd;
+  f[26] = xv * k.kx;
   for (int c = kFeatUsed; c < kFeat; ++c) f[c] = 0.f;
 }
 
