@@ -53,11 +53,29 @@ struct Hit {
 // Segment p0 -> p1 against triangle f (collisions.cl:21-76): on a hit,
 // returns true with the intersection, the normal oriented along the motion
 // and the penetration depth |p1 - intersection|.
+//
+// Most segments are far from most triangles' planes, and those are rejected
+// first without the divisions and square roots. Negating the normal negates
+// both dot products below exactly, so r = dot(n, v0 - p0) / dot(n, seg) does
+// not depend on the orientation step; the reference rejects whenever its
+// quotient is outside [0, 1], which |num| > 2 |denom| guarantees (the rounded
+// quotient is then at least 2 in magnitude), and so do opposite signs while
+// the quotient is far above the underflow range (a quotient that rounds to
+// -0 would pass the reference's 0 <= r). Every other segment takes the
+// reference's full test.
 __device__ inline bool face_hit(uint32_t f, V3 p0, V3 p1, V3 seg, float seg_len, const Mesh& m,
                                 V3& isect, V3& nrm, float& depth) {
   nrm = v3(m.normals[3 * f + 0], m.normals[3 * f + 1], m.normals[3 * f + 2]);
-  if (dot(nrm, seg) / (len(nrm) * seg_len) <= 0.f) nrm = v3(-nrm.x, -nrm.y, -nrm.z);
   const float* a = m.verts + 3 * m.idx[3 * f + 0];
+  {
+    const float den0 = dot(nrm, seg);
+    const float num0 = dot(nrm, v3(a[0], a[1], a[2]) - p0);
+    if (den0 == 0.f) return false;
+    const float an = fabsf(num0), ad = fabsf(den0);
+    if (an > 2.f * ad) return false;
+    if ((num0 < 0.f) != (den0 < 0.f) && an > ad * 0x1p-60f) return false;
+  }
+  if (dot(nrm, seg) / (len(nrm) * seg_len) <= 0.f) nrm = v3(-nrm.x, -nrm.y, -nrm.z);
   const float* b = m.verts + 3 * m.idx[3 * f + 1];
   const float* c = m.verts + 3 * m.idx[3 * f + 2];
   V3 v0 = v3(a[0], a[1], a[2]);
