@@ -22,7 +22,7 @@ import numpy as np
 CANDIDATES = (64, 128, 256, 512)
 # Stages that run over the work-group table (stage order of clsph_time_stage).
 TUNED = {"hash": 1, "density": 4, "forces": 5, "advect": 6}
-DEFAULT = [256, 256, 256, 256, 512, 256, 256]  # the state's own defaults
+DEFAULT = [256, 256, 256, 256, 512, 512, 256]  # the state's own defaults
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "mi355x.json")
 
 

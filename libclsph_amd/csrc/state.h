@@ -84,8 +84,9 @@ struct clsph_state {
   clsph::BlockDesc* blocks[clsph::kNumBlockSizes] = {};
   uint32_t nblocks[clsph::kNumBlockSizes] = {};
   // indices into kBlockSizes: 256 threads, 512 for density (two 80 KB
-  // halo-staging work-groups per CU, measured fastest at 1M particles)
-  int stage_bs[kNumStages] = {2, 2, 2, 2, 3, 2, 2};
+  // halo-staging work-groups per CU) and forces, measured fastest at 1M
+  // particles
+  int stage_bs[kNumStages] = {2, 2, 2, 2, 3, 3, 2};
 
   // Collision meshes of all scenes, concatenated (uploaded once;
   // sph_simulation.cpp:296-319 re-uploads it every substep).
