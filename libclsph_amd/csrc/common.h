@@ -97,6 +97,12 @@ __host__ __device__ inline uint32_t bitlen(uint32_t v) {
   return b;
 }
 
+// The learned surrogate's node inputs of a particle: nu = [rho / rho0 - 1,
+// asinh(p / K) / 4].
+__device__ inline float2 node_in(float rho, float p, float rho0, float K) {
+  return make_float2(rho / rho0 - 1.f, asinhf(p / K) * 0.25f);
+}
+
 // The particle a thread of a per-particle kernel handles, or UINT32_MAX when
 // the thread is past the end of its scene.
 __device__ inline uint32_t particle_of(const BlockDesc* __restrict__ blocks,

@@ -118,9 +118,10 @@ void forward(clsph_nn* m, const float4* acc0, float gx, float gy, float gz);
 // (then acc must hold the forces stage's result for the particles whose list
 // overflowed: launch_forces_overflow). With `out` non-null the full output
 // is written there as well. With `ev` (2 events) the stream gets events
-// before and after the kernel.
+// before and after the kernel. With `nu_ready` the node inputs m->nu are
+// already current (the density stage wrote them).
 void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float* out, bool own_forces,
-                   hipEvent_t* ev = nullptr);
+                   hipEvent_t* ev = nullptr, bool nu_ready = false);
 // Loss against m->target and gradients into m->grad (zeroed first).
 void backward(clsph_nn* m);
 void adam(clsph_nn* m, float lr, float beta1, float beta2, float eps);

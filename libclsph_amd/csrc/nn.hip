@@ -230,9 +230,6 @@ __global__ __launch_bounds__(256) void k_scan_add(uint32_t* __restrict__ off,
 }
 
 // ---------------------------------------------------------------------------
-__device__ inline float2 node_in(float rho, float p, float rho0, float K) {
-  return make_float2(rho / rho0 - 1.f, asinhf(p / K) * 0.25f);
-}
 
 // src / dst of every edge (dst-major, the density kernel's neighbour order).
 __global__ __launch_bounds__(256) void k_graph(const uint4* __restrict__ nbr,
@@ -1069,14 +1066,15 @@ void forward(clsph_nn* m, const float4* acc0, float gx, float gy, float gz) {
 }
 
 void forward_fused(clsph_nn* m, clsph_state* st, float4* acc, float* out, bool own_forces,
-                   hipEvent_t* ev) {
+                   hipEvent_t* ev, bool nu_ready) {
   hipStream_t s = m->ctx->stream;
   uint32_t n = st->n;
   m->n = n;
   const DevParams& d = st->host_dps[0];
   if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL(k_node_inputs, grid_for(n), dim3(256), 0, s, st->rho_s, st->prs_s, n,
-                     d.p.fluid_density, d.p.K, (float2*)m->nu);
+  if (!nu_ready)
+    hipLaunchKernelGGL(k_node_inputs, grid_for(n), dim3(256), 0, s, st->rho_s, st->prs_s, n,
+                       d.p.fluid_density, d.p.K, (float2*)m->nu);
   // persistent grid: the work-groups that fit on the device at once (the
   // kernel's registers and LDS allow two per compute unit) stage the weights
   // once each

@@ -124,9 +124,9 @@ int ensure_learned_graph(clsph_state* st, clsph_nn* m) {
   launch_sort(st);
   launch_permute(st);
   launch_cell_table(st);
-  launch_density(st);
+  launch_density(st, (float2*)m->nu);
   launch_forces_overflow(st);
-  nn::forward_fused(m, st, st->acc_s, nullptr, true);
+  nn::forward_fused(m, st, st->acc_s, nullptr, true, nullptr, true);
   launch_advect(st);
   if (st->ids) launch_track_ids(st);
   hipError_t le = hipGetLastError();

@@ -253,9 +253,10 @@ struct DensityHalos {
 // Density and pressure (sph.cl:9-40, forces.cl:15-43) of the particles
 // [first, first + 64 WAVES) of one scene, with each particle's neighbour
 // list (self excluded) in visiting order into nbr[q * n + i], its count into
-// nbr_count[i] and the force inputs fab[2i] = (x, p / rho^2), fab[2i + 1] =
-// (v, m / rho). The work-group first stages, for each distinct cell (side 2h) its
-// particles fall in, the cell's 4x4x4 half-cell halo in LDS (coalesced runs
+// nbr_count[i], the force inputs fab[2i] = (x, p / rho^2), fab[2i + 1] =
+// (v, m / rho), and with `nu` the learned surrogate's node inputs. The
+// work-group first stages, for each distinct cell (side 2h) its particles
+// fall in, the cell's 4x4x4 half-cell halo in LDS (coalesced runs
 // of the sorted positions); particles then walk their 3x3x3 half-cells
 // there. Cells beyond the staging capacity, and every particle when the
 // table is unusable, walk the sorted positions in HBM instead, with the same
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_density(const float4* __restrict
                                                         float4* __restrict__ fab,
                                                         uint4* __restrict__ nbr,
                                                         uint32_t* __restrict__ nbr_count,
-                                                        uint32_t n) {
+                                                        float2* __restrict__ nu, uint32_t n) {
   using Halos = DensityHalos<WAVES>;
   static_assert(Halos::kHalos <= 2u * WAVES, "at most two (halo, half-cell) entries per thread");
   __shared__ Halos sh;
@@ -397,6 +398,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_density(const float4* __restrict
   float4 vi = vel[i];
   fab[2u * i] = make_float4(xi.x, xi.y, xi.z, p / (d * d));
   fab[2u * i + 1u] = make_float4(vi.x, vi.y, vi.z, m / d);
+  if (nu) nu[i] = node_in(d, p, dp->p.fluid_density, dp->p.K);
 }
 
 // Pair terms of compute_internal_forces_with_grid (forces.cl:60-101) for one
@@ -545,13 +547,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_forces(const float4* __restrict__
   acc_out[i] = make_float4(f.x / rho_i + g.s[0], f.y / rho_i + g.s[1], f.z / rho_i + g.s[2], 0.f);
 }
 
-void launch_density(clsph_state* s) {
+void launch_density(clsph_state* s, float2* nu) {
   if (s->n == 0) return;
   const int bs = s->bs_of(kStageDensity);
   auto go = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, dim3(s->nblocks_of(kStageDensity)), dim3(bs), 0, s->ctx->stream,
                        s->pos_s, s->vel_s, s->key_b, s->table, s->table_of(kStageDensity), s->dps,
-                       s->bp, s->rho_s, s->prs_s, s->fab, s->nbr, s->nbr_count, s->n);
+                       s->bp, s->rho_s, s->prs_s, s->fab, s->nbr, s->nbr_count, nu, s->n);
   };
   switch (bs) {
     case 64: go(k_density<1>); break;

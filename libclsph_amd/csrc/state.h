@@ -160,7 +160,8 @@ void launch_pack_aos(clsph_state* s, clsph_particle* out);
 void launch_sort(clsph_state* s);
 uint32_t sort_tiles_for(uint32_t n);
 // sph.hip
-void launch_density(clsph_state* s);
+// With `nu`, also the learned surrogate's node inputs of every particle.
+void launch_density(clsph_state* s, float2* nu = nullptr);
 void launch_forces(clsph_state* s);
 // The forces stage for the particles whose neighbour list overflowed only.
 void launch_forces_overflow(clsph_state* s);
