@@ -305,8 +305,10 @@ uint32_t clsph_make_block(clsph_particle* out, uint32_t n, float spacing,
                           const float origin[3], const float extent[3],
                           float jitter, uint64_t seed);
 /* Smoothing-kernel constants for support radius h, with the reference's
- * expressions (sph_simulation.cpp:499-505: float pow(h, 9.f) / pow(h, 6.f),
- * double M_PI arithmetic, stored as float). */
+ * expressions (sph_simulation.cpp:499-505). There the unqualified
+ * pow(h, 9.f) / pow(h, 6.f) resolve to ::pow(double, double) under
+ * g++/libstdc++, so h^9, h^6, M_PI and every quotient are double and each
+ * constant is rounded once to float on store. */
 void clsph_kernel_terms(float h, clsph_precomputed_kernel_values* out);
 /* Heightfield over [x0, x0+size] x [z0, z0+size]: res x res quads, 2
  * triangles each (2 res^2 faces, (res+1)^2 vertices), heights

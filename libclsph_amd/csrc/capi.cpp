@@ -1137,13 +1137,17 @@ uint32_t clsph_make_column(clsph_particle* out, uint32_t n, float spacing, const
 
 void clsph_kernel_terms(float h, clsph_precomputed_kernel_values* out) {
   if (!out) return;
+  // sph_simulation.cpp:499-505 writes pow(h, 9.f) unqualified after
+  // <cmath>; under g++/libstdc++ that names ::pow(double, double), so the
+  // powers and every quotient are double, rounded once to float on store.
   const double pi = 3.14159265358979323846;
-  float h9 = powf(h, 9.f), h6 = powf(h, 6.f);
-  out->poly_6 = (float)(315.f / (64.f * pi * h9));
-  out->poly_6_gradient = (float)(-945.f / (32.f * pi * h9));
-  out->poly_6_laplacian = (float)(-945.f / (32.f * pi * h9));
-  out->spiky = (float)(-45.f / (pi * h6));
-  out->viscosity = (float)(45.f / (pi * h6));
+  const double hd = (double)h;
+  const double h9 = ::pow(hd, 9.0), h6 = ::pow(hd, 6.0);
+  out->poly_6 = (float)(315.0 / (64.0 * pi * h9));
+  out->poly_6_gradient = (float)(-945.0 / (32.0 * pi * h9));
+  out->poly_6_laplacian = (float)(-945.0 / (32.0 * pi * h9));
+  out->spiky = (float)(-45.0 / (pi * h6));
+  out->viscosity = (float)(45.0 / (pi * h6));
 }
 
 void clsph_make_box(const float lo[3], const float hi[3], float* vertices, uint32_t* indices,

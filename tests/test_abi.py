@@ -35,15 +35,35 @@ def test_library_exports_every_declared_symbol():
         getattr(native.lib, n)  # resolvable through ctypes
 
 
-def test_kernel_terms_closed_form():
-    h = 0.0123
-    t = native.kernel_terms(h)
-    hf = float(np.float32(h))
-    np.testing.assert_allclose(t["poly_6"], 315 / (64 * math.pi * hf ** 9), rtol=1e-6)
-    np.testing.assert_allclose(t["poly_6_gradient"], -945 / (32 * math.pi * hf ** 9), rtol=1e-6)
-    np.testing.assert_allclose(t["poly_6_laplacian"], -945 / (32 * math.pi * hf ** 9), rtol=1e-6)
-    np.testing.assert_allclose(t["spiky"], -45 / (math.pi * hf ** 6), rtol=1e-6)
-    np.testing.assert_allclose(t["viscosity"], 45 / (math.pi * hf ** 6), rtol=1e-6)
+def kernel_terms_f64(h):
+    """sph_simulation.cpp:499-505 as g++/libstdc++ compiles it: pow(h, 9.f)
+    unqualified is ::pow(double, double), the arithmetic is double, and each
+    value is rounded once to float (numpy's float32 cast rounds to nearest)."""
+    hd = float(np.float32(h))
+    h9, h6 = math.pow(hd, 9.0), math.pow(hd, 6.0)
+    return {
+        "poly_6": np.float32(315.0 / (64.0 * math.pi * h9)),
+        "poly_6_gradient": np.float32(-945.0 / (32.0 * math.pi * h9)),
+        "poly_6_laplacian": np.float32(-945.0 / (32.0 * math.pi * h9)),
+        "spiky": np.float32(-45.0 / (math.pi * h6)),
+        "viscosity": np.float32(45.0 / (math.pi * h6)),
+    }
+
+
+def test_kernel_terms_bit_exact_over_many_h():
+    rng = np.random.default_rng(3)
+    hs = np.concatenate([rng.uniform(1e-3, 0.2, 4000), np.float32([0.0123, 0.04, 9.921e-3])])
+    powf_differs = 0
+    for h in hs.astype(np.float32):
+        t = native.kernel_terms(float(h))
+        want = kernel_terms_f64(h)
+        for k, v in want.items():
+            assert t[k].tobytes() == v.tobytes(), (float(h), k, float(t[k]), float(v))
+        # float pow would round h^9 to float first: it differs for many h,
+        # so this test tells the two readings of the reference apart
+        h9f = np.float32(math.pow(float(h), 9.0))
+        powf_differs += np.float32(315.0 / (64.0 * math.pi * float(h9f))) != want["poly_6"]
+    assert powf_differs > len(hs) // 10
 
 
 def test_make_box_inward_unit_normals():
