@@ -17,7 +17,7 @@ OUT = os.path.join(HERE, "libclsph_hip.so")
 SOURCES = ["grid.hip", "sort.hip", "sph.hip", "advect.hip", "nn.hip", "capi.cpp", "nn_capi.cpp"]
 API = os.path.join(HERE, "api")
 API_SOURCES = ["sph_simulation.cpp", "scene.cpp", "json.cpp",
-               "file_save_delegates/houdini_file_saver.cpp"]
+               "file_save_delegates/houdini_file_saver.cpp", "util/cl_boilerplate.cpp"]
 API_OUT = os.path.join(HERE, "libclsph.so")
 CLI_SRC = os.path.join(os.path.dirname(HERE), "examples", "particles.cpp")
 CLI_OUT = os.path.join(HERE, "bin", "clsph_particles")

@@ -534,11 +534,14 @@ int read_u32(clsph_state* st, const uint32_t* src, uint32_t* dst, size_t count) 
 
 int ensure_graph(clsph_state* st) {
   if (!st->graph_dirty && st->graph) return CLSPH_OK;
+  hipStream_t s = st->ctx->stream;
   if (st->graph) {
+    // Replays of the old executable may still be queued on the stream: let
+    // them finish before it is destroyed.
+    HIPCK(st->ctx, hipStreamSynchronize(s));
     (void)hipGraphExecDestroy(st->graph);
     st->graph = nullptr;
   }
-  hipStream_t s = st->ctx->stream;
   hipGraph_t g = nullptr;
   HIPCK(st->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   enqueue_step(st);

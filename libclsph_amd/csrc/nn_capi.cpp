@@ -112,11 +112,13 @@ void enqueue_density_stages(clsph_state* st) {
 int ensure_learned_graph(clsph_state* st, clsph_nn* m) {
   if (st->lgraph && !st->lgraph_dirty && st->lgraph_serial == m->serial && st->lgraph_n == st->n)
     return CLSPH_OK;
+  hipStream_t s = st->ctx->stream;
   if (st->lgraph) {
+    // Queued replays of the old executable finish before it is destroyed.
+    HIPCK(st->ctx, hipStreamSynchronize(s));
     (void)hipGraphExecDestroy(st->lgraph);
     st->lgraph = nullptr;
   }
-  hipStream_t s = st->ctx->stream;
   hipGraph_t graph = nullptr;
   HIPCK(st->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   launch_grid_params(st);

@@ -14,6 +14,7 @@
 #include "../../libclsph_amd/api/common/util.h"
 #include "../../libclsph_amd/api/file_save_delegates/houdini_file_saver.h"
 #include "../../libclsph_amd/api/sph_simulation.h"
+#include "../../libclsph_amd/api/util/cl_boilerplate.h"
 
 static int fails = 0;
 #define EXPECT(c)                                                      \
@@ -63,6 +64,12 @@ static std::string ostream_geo(const particle* p, unsigned n, float mass) {
 }
 
 int main(int argc, char** argv) {
+  // `read <file>`: prints readKernelFile's result (its error exits are
+  // checked by the caller from the exit status and the output)
+  if (argc > 2 && std::string(argv[1]) == "read") {
+    std::cout << readKernelFile(argv[2]);
+    return 0;
+  }
   std::string dir = argc > 1 ? argv[1] : ".";
 
   // ---- load_settings: fields and derived values (sph_simulation.cpp:405-506)

@@ -47,11 +47,9 @@ def cloud(n, seed, dup=0, lo=0.2, hi=0.8):
                         sc.indices)
 
 
-def run_gpu_stages(ctx, sc):
-    st = native.State(ctx, sc.n, table_cap=1 << 22)
-    st.set_params(sc.params, sc.terms)
-    st.scene(sc.normals, sc.vertices, sc.indices)
-    st.upload(sc.particles)
+def stage_outputs(st):
+    """Runs the six stages and advection of one substep on state `st` from
+    its current particles; returns every stage's readback."""
     out = {}
     st.stage("grid")
     out["grid"] = st.grid()
@@ -69,8 +67,38 @@ def run_gpu_stages(ctx, sc):
     st.stage("advect")
     out["advected"] = st.download()
     out["capped"] = st.bounce_capped()
+    return out
+
+
+def run_gpu_stages(ctx, sc):
+    st = native.State(ctx, sc.n, table_cap=1 << 22)
+    st.set_params(sc.params, sc.terms)
+    st.scene(sc.normals, sc.vertices, sc.indices)
+    st.upload(sc.particles)
+    out = stage_outputs(st)
     st.close()
     return out
+
+
+def boundary_cloud(seed, h=0.04):
+    """Particles exactly on cell faces of the padded grid and one float32 ulp
+    either side of them (the cell-id truncation's edge cases), plus random
+    ones; an anchor and a far-corner particle fix the bounding box."""
+    rng = np.random.default_rng(seed)
+    f = np.float32
+    L = f(h) * f(2)
+    anchor = rng.uniform(0.1, 0.3, 3).astype(f)
+    mn = anchor - L * f(2)
+    pts = [anchor, anchor + f(0.5)]
+    for _ in range(1500):
+        b = (mn + rng.integers(3, 8, 3).astype(f) * L).astype(f)
+        side = rng.integers(-1, 2, 3)
+        pts.append(np.where(side < 0, np.nextafter(b, f(-1)),
+                            np.where(side > 0, np.nextafter(b, f(2)), b)).astype(f))
+    pts += list(rng.uniform(anchor, anchor + f(0.5), (1000, 3)).astype(f))
+    sc = cloud(len(pts), seed)
+    sc.particles["position"][:, :3] = np.array(pts, dtype=f)
+    return sc
 
 
 CASES = {
@@ -83,6 +111,7 @@ CASES = {
     "cloud4097": lambda: cloud(4097, 5),
     # 300 particles inside one h-sphere: more neighbours than a list holds
     "dense300": lambda: cloud(300, 6, lo=0.49, hi=0.51),
+    "boundary2502": lambda: boundary_cloud(8),
 }
 
 
@@ -93,7 +122,12 @@ def test_stage_parity(ctx, case):
     ref = oracle.stages(sc.particles, sc.params, sc.terms, sc.normals, sc.vertices, sc.indices)
     assert ref.grid_rc == 0
     g = run_gpu_stages(ctx, sc)
+    check_against_oracle(sc.particles, sc.params, g, ref)
 
+
+def check_against_oracle(parts, params, g, ref):
+    """Compares the readbacks of stage_outputs with the oracle's stages of
+    the same substep from the same particles `parts`."""
     # grid parameters: bit-exact
     for f in ("min_point", "max_point"):
         np.testing.assert_array_equal(g["grid"][f][:3], ref.params[f][:3], err_msg=f)
@@ -105,7 +139,7 @@ def test_stage_parity(ctx, case):
     # stable sorted order: bit-exact keys and the same permutation
     np.testing.assert_array_equal(g["sorted_keys"], ref.sorted["grid_index"])
     np.testing.assert_array_equal(g["perm"], np.argsort(ref.hashed["grid_index"], kind="stable"))
-    np.testing.assert_array_equal(sc.particles["position"][g["perm"]], ref.sorted["position"])
+    np.testing.assert_array_equal(parts["position"][g["perm"]], ref.sorted["position"])
     # cell table: bit-exact
     np.testing.assert_array_equal(g["table"], ref.table)
 
@@ -126,7 +160,7 @@ def test_stage_parity(ctx, case):
     # advection: positions and velocities after the substep
     rpos = ref.advected["position"][:, :3].astype(np.float64)
     gpos = g["advected"]["position"][:, :3].astype(np.float64)
-    dt = float(sc.params["time_delta"])
+    dt = float(params["time_delta"])
     # a 1e-4*max|a| force difference moves a particle by at most that * dt^2
     tol = FORCE_TOL * amax * dt * dt + 4e-7
     assert np.max(np.abs(gpos - rpos)) <= tol

@@ -70,6 +70,67 @@ def test_grid_params_invariants(n, seed):
             assert 1 <= c[a] <= gs[a] - 2
 
 
+def np32_grid(pos, h):
+    """The grid of sph_simulation.cpp:201-252 and the cell ids of
+    grid.cl:43-67 restated in numpy float32 arithmetic (every operation
+    rounded to float32 as the reference's float variables are): bounding box
+    from float(INT_MAX) / float(INT_MIN) start values, two cells of padding,
+    truncated dimensions, the Morton count and each particle's truncated
+    cell coordinates."""
+    f = np.float32
+    L = f(h) * f(2)
+    mn = np.full(3, f(2147483647), dtype=f)
+    mx = np.full(3, f(-2147483648), dtype=f)
+    mn = np.minimum(mn, pos.min(axis=0))
+    mx = np.maximum(mx, pos.max(axis=0))
+    mn = mn - L * f(2)
+    mx = mx + L * f(2)
+    gs = ((mx - mn) / L).astype(np.uint32)
+    cell = ((pos - mn) / L).astype(np.uint32)
+    keys = np.array([bit_morton(int(x), int(y), int(z)) for x, y, z in cell], dtype=np.uint32)
+    return mn, mx, gs, bit_morton(*[int(v) for v in gs]), keys
+
+
+def boundary_cloud(seed, h=0.03):
+    """Particles exactly on cell faces of the padded grid, one float32 ulp
+    either side of them, and random ones, all inside the box an anchor
+    particle and a far corner particle fix."""
+    rng = np.random.default_rng(seed)
+    f = np.float32
+    L = f(h) * f(2)
+    anchor = rng.uniform(0.1, 0.4, 3).astype(f)
+    mn = anchor - L * f(2)
+    pts = [anchor, anchor + f(0.3)]
+    for _ in range(300):
+        k = rng.integers(3, 12, 3)
+        b = (mn + k.astype(f) * L).astype(f)
+        side = rng.integers(-1, 2, 3)
+        b = np.where(side < 0, np.nextafter(b, f(-1)), np.where(side > 0, np.nextafter(b, f(2)), b))
+        pts.append(b.astype(f))
+    pts += list(rng.uniform(anchor, anchor + f(0.3), (200, 3)).astype(f))
+    parts = layout.particles(len(pts))
+    parts["position"][:, :3] = np.array(pts, dtype=f)
+    p = scenes.fluid_params(len(pts), h, 1e-3)
+    return parts, p
+
+
+@pytest.mark.parametrize("kind,seed", [("random", 11), ("random", 12), ("boundary", 13),
+                                       ("boundary", 14), ("boundary", 15)])
+def test_grid_and_cell_ids_bit_exact_against_float32_restatement(kind, seed):
+    if kind == "random":
+        parts, p = random_cloud(2000, seed)
+    else:
+        parts, p = boundary_cloud(seed)
+    assert oracle.grid_params(parts, p) == 0
+    mn, mx, gs, count, keys = np32_grid(parts["position"][:, :3].copy(), p["h"])
+    np.testing.assert_array_equal(p["min_point"][:3].view(np.uint32), mn.view(np.uint32))
+    np.testing.assert_array_equal(p["max_point"][:3].view(np.uint32), mx.view(np.uint32))
+    np.testing.assert_array_equal([p["grid_size_x"], p["grid_size_y"], p["grid_size_z"]], gs)
+    assert int(p["grid_cell_count"]) == count
+    hashed = oracle.locate_in_grid(parts, p)
+    np.testing.assert_array_equal(hashed["grid_index"], keys)
+
+
 def test_grid_dimension_assert():
     parts, p = random_cloud(2, 0)
     parts["position"][1, 0] = 100.0  # 100 m away with h = 3 cm: > 1024 cells
