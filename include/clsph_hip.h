@@ -292,6 +292,18 @@ int clsph_read_ids(clsph_state* st, uint32_t* ids, uint32_t n);
 int clsph_nn_gemm(clsph_ctx* ctx, int trans_a, int trans_b, int accumulate, const uint16_t* a,
                   const uint16_t* b, float* c, const float* bias, uint32_t m, uint32_t n,
                   uint32_t k);
+/* clsph_nn_gemm without any device allocation: no counterpart in the
+ * reference. For callers that cannot allocate device memory (harnesses that
+ * forbid hipMalloc, tight loops that must not allocate). The device copies of
+ * a, b, c and bias go to a static workspace of 8 MiB per GPU, held until the
+ * process ends; a call whose operands (each at a 256-byte aligned offset)
+ * need more fails with CLSPH_EINVAL. Results equal clsph_nn_gemm's. Every
+ * context on a GPU shares that GPU's workspace: calls on different contexts
+ * of one GPU that overlap in time race; calls on one context do not (each
+ * waits for its stream). */
+int clsph_nn_gemm_noalloc(clsph_ctx* ctx, int trans_a, int trans_b, int accumulate,
+                          const uint16_t* a, const uint16_t* b, float* c, const float* bias,
+                          uint32_t m, uint32_t n, uint32_t k);
 
 /* ---- scene builders (host only; no device needed) ---- */
 /* Dam-break block: a cubic lattice of `n` particles at `spacing` with

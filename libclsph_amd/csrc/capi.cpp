@@ -10,6 +10,7 @@
 
 #include "../../include/clsph_hip.h"
 #include "hostutil.h"
+#include "nn.h"
 #include "state.h"
 
 using namespace clsph;
@@ -617,6 +618,13 @@ int clsph_init(int device, clsph_ctx** out) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->cus = prop.multiProcessorCount;
+  hipError_t se = clsph::nn::scratch_address(device, &ctx->scratch);
+  if (se != hipSuccess) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return fail(nullptr, CLSPH_EDEVICE,
+                std::string("device workspace address: ") + hipGetErrorString(se));
+  }
   *out = ctx;
   return CLSPH_OK;
 }

@@ -140,6 +140,11 @@ void launch_target_mean(clsph_nn* m, clsph_state* learner, clsph_state* teacher,
 // synchronisation) = {max |rho_a - rho_b|, sum |rho_a - rho_b|}.
 void launch_compare(clsph_nn* m, clsph_state* a, clsph_state* b, float res[2]);
 void fill(float* x, size_t count, float v, hipStream_t s);
+// Bytes of the static device workspace of clsph_nn_gemm_noalloc.
+constexpr size_t kScratchBytes = size_t(8) << 20;
+// Device address of that workspace on `device` (which must be current),
+// looked up on the first call for the device.
+hipError_t scratch_address(int device, void** ptr);
 // The bf16 matrix-core GEMM (see nn.hip) for direct use and testing.
 void gemm_public(hipStream_t s, bool ta, bool tb, bool add, const uint16_t* A, int lda,
                  const uint16_t* B, int ldb, float* C, int ldc, int M, int N, int K, float alpha,

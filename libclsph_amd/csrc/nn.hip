@@ -4,6 +4,9 @@
 // backward passes are sequences of these launches on the context stream.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <vector>
+
 #include "nn.h"
 
 namespace clsph {
@@ -1186,6 +1189,30 @@ void launch_target(clsph_nn* m, clsph_state* a, clsph_state* t, float dt) {
   hipLaunchKernelGGL(k_invert, grid_for(n), dim3(256), 0, s, t->ids, n, m->loc);
   hipLaunchKernelGGL(k_target, grid_for(n), dim3(256), 0, s, a->val_b, a->vh, m->loc, t->vh,
                      a->acc_s, n, 1.f / dt, m->target);
+}
+
+// Workspace of clsph_nn_gemm_noalloc: the device copies of its operands.
+// Every GPU the library runs on holds one copy of this array until the
+// process ends. Host code never passes the array's name to a kernel or a
+// copy: the name is a host-side stand-in, and the device address comes from
+// hipGetSymbolAddress below, per device.
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+hipError_t scratch_address(int device, void** ptr) {
+  static std::mutex lock;
+  static std::vector<void*> addr;  // per device, looked up once
+  std::lock_guard<std::mutex> g(lock);
+  if (device < 0) return hipErrorInvalidDevice;
+  if ((size_t)device >= addr.size()) addr.resize((size_t)device + 1, nullptr);
+  if (!addr[device]) {
+    // the caller has made `device` current
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+  }
+  *ptr = addr[device];
+  return hipSuccess;
 }
 
 void gemm_public(hipStream_t s, bool ta, bool tb, bool add, const uint16_t* A, int lda,

@@ -537,41 +537,88 @@ int clsph_nn_compare_density(clsph_nn* m, clsph_state* a, clsph_state* b, float*
   return CLSPH_OK;
 }
 
-int clsph_nn_gemm(clsph_ctx* ctx, int trans_a, int trans_b, int accumulate, const uint16_t* a,
-                  const uint16_t* b, float* c, const float* bias, uint32_t m, uint32_t n,
-                  uint32_t k) {
-  if (!ctx || !a || !b || !c || !m || !n || !k) return fail(ctx, CLSPH_EINVAL, "clsph_nn_gemm: bad argument");
+namespace {
+
+// Checks the shape of a clsph_nn_gemm call.
+int gemm_check(clsph_ctx* ctx, const char* who, int trans_a, int trans_b, const uint16_t* a,
+               const uint16_t* b, float* c, uint32_t m, uint32_t n, uint32_t k) {
+  if (!ctx || !a || !b || !c || !m || !n || !k)
+    return fail(ctx, CLSPH_EINVAL, std::string(who) + ": bad argument");
   // contiguous dimensions must hold whole 8-element (16-byte) vectors
   if ((trans_a ? m : k) % 8 || (trans_b ? k : n) % 8)
-    return fail(ctx, CLSPH_EINVAL, "clsph_nn_gemm: contiguous dimensions must be multiples of 8");
+    return fail(ctx, CLSPH_EINVAL,
+                std::string(who) + ": contiguous dimensions must be multiples of 8");
+  return CLSPH_OK;
+}
+
+// Copies the host operands into the device buffers da, db, dc (and dbias),
+// runs the product there and copies c back; waits for the stream.
+int gemm_staged(clsph_ctx* ctx, const char* who, int trans_a, int trans_b, int accumulate,
+                const uint16_t* a, const uint16_t* b, float* c, const float* bias, uint32_t m,
+                uint32_t n, uint32_t k, uint16_t* da, uint16_t* db, float* dc, float* dbias) {
   size_t sa = (size_t)m * k, sb = (size_t)k * n, sc = (size_t)m * n;
-  uint16_t *da = nullptr, *db = nullptr;
-  float *dc = nullptr, *dbias = nullptr;
-  int rc = dalloc(ctx, &da, sa);
-  if (!rc) rc = dalloc(ctx, &db, sb);
-  if (!rc) rc = dalloc(ctx, &dc, sc);
-  if (!rc && bias) rc = dalloc(ctx, &dbias, n);
   hipStream_t s = ctx->stream;
-  auto done = [&](int code) {
-    (void)hipStreamSynchronize(s);
-    dfree(da), dfree(db), dfree(dc), dfree(dbias);
-    return code;
-  };
-  if (rc) return done(rc);
   if (hipMemcpyAsync(da, a, 2 * sa, hipMemcpyHostToDevice, s) != hipSuccess ||
       hipMemcpyAsync(db, b, 2 * sb, hipMemcpyHostToDevice, s) != hipSuccess ||
       hipMemcpyAsync(dc, c, 4 * sc, hipMemcpyHostToDevice, s) != hipSuccess ||
       (bias && hipMemcpyAsync(dbias, bias, 4 * n, hipMemcpyHostToDevice, s) != hipSuccess))
-    return done(fail(ctx, CLSPH_EDEVICE, "clsph_nn_gemm: copy failed"));
+    return fail(ctx, CLSPH_EDEVICE, std::string(who) + ": copy failed");
   int lda = trans_a ? (int)m : (int)k, ldb = trans_b ? (int)k : (int)n;
   // a non-accumulating product over a long K runs as one chunk
   nn::gemm_public(s, trans_a, trans_b, accumulate, da, lda, db, ldb, dc, (int)n, (int)m, (int)n,
-                  (int)k, 1.f, dbias, nullptr);
-  rc = check_launch(ctx);
-  if (rc) return done(rc);
+                  (int)k, 1.f, bias ? dbias : nullptr, nullptr);
+  int rc = check_launch(ctx);
+  if (rc) return rc;
   if (hipMemcpyAsync(c, dc, 4 * sc, hipMemcpyDeviceToHost, s) != hipSuccess)
-    return done(fail(ctx, CLSPH_EDEVICE, "clsph_nn_gemm: copy back failed"));
-  return done(CLSPH_OK);
+    return fail(ctx, CLSPH_EDEVICE, std::string(who) + ": copy back failed");
+  HIPCK(ctx, hipStreamSynchronize(s));
+  return CLSPH_OK;
+}
+
+size_t round256(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+}  // namespace
+
+int clsph_nn_gemm(clsph_ctx* ctx, int trans_a, int trans_b, int accumulate, const uint16_t* a,
+                  const uint16_t* b, float* c, const float* bias, uint32_t m, uint32_t n,
+                  uint32_t k) {
+  int rc = gemm_check(ctx, "clsph_nn_gemm", trans_a, trans_b, a, b, c, m, n, k);
+  if (rc) return rc;
+  size_t sa = (size_t)m * k, sb = (size_t)k * n, sc = (size_t)m * n;
+  uint16_t *da = nullptr, *db = nullptr;
+  float *dc = nullptr, *dbias = nullptr;
+  rc = dalloc(ctx, &da, sa);
+  if (!rc) rc = dalloc(ctx, &db, sb);
+  if (!rc) rc = dalloc(ctx, &dc, sc);
+  if (!rc && bias) rc = dalloc(ctx, &dbias, n);
+  if (!rc)
+    rc = gemm_staged(ctx, "clsph_nn_gemm", trans_a, trans_b, accumulate, a, b, c, bias, m, n, k,
+                     da, db, dc, dbias);
+  (void)hipStreamSynchronize(ctx->stream);
+  dfree(da), dfree(db), dfree(dc), dfree(dbias);
+  return rc;
+}
+
+int clsph_nn_gemm_noalloc(clsph_ctx* ctx, int trans_a, int trans_b, int accumulate,
+                          const uint16_t* a, const uint16_t* b, float* c, const float* bias,
+                          uint32_t m, uint32_t n, uint32_t k) {
+  int rc = gemm_check(ctx, "clsph_nn_gemm_noalloc", trans_a, trans_b, a, b, c, m, n, k);
+  if (rc) return rc;
+  if (!ctx->scratch) return fail(ctx, CLSPH_EDEVICE, "clsph_nn_gemm_noalloc: no workspace");
+  // a, b, c and bias at 256-byte aligned offsets of the static workspace
+  size_t oa = 0;
+  size_t ob = oa + round256(2 * (size_t)m * k);
+  size_t oc = ob + round256(2 * (size_t)k * n);
+  size_t obias = oc + round256(4 * (size_t)m * n);
+  size_t total = obias + (bias ? 4 * (size_t)n : 0);
+  if (total > nn::kScratchBytes)
+    return fail(ctx, CLSPH_EINVAL,
+                "clsph_nn_gemm_noalloc: the operands need " + std::to_string(total) +
+                    " bytes of device workspace; it holds " + std::to_string(nn::kScratchBytes));
+  uint8_t* w = (uint8_t*)ctx->scratch;
+  return gemm_staged(ctx, "clsph_nn_gemm_noalloc", trans_a, trans_b, accumulate, a, b, c, bias,
+                     m, n, k, (uint16_t*)(w + oa), (uint16_t*)(w + ob), (float*)(w + oc),
+                     (float*)(w + obias));
 }
 
 }  // extern "C"

@@ -18,6 +18,9 @@ struct clsph_ctx {
   hipStream_t stream = nullptr;
   std::string last_error;
   int cus = 256;  // compute units on the device
+  // Device address of the static workspace of clsph_nn_gemm_noalloc on this
+  // context's device (shared by every context on the device).
+  void* scratch = nullptr;
 };
 
 // Stage indices (clsph_time_stage order).

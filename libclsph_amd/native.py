@@ -113,6 +113,8 @@ def _load():
         "clsph_track_ids": (_i32, [_vp, _i32]),
         "clsph_read_ids": (_i32, [_vp, _vp, _u32]),
         "clsph_nn_gemm": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _u32, _u32, _u32]),
+        "clsph_nn_gemm_noalloc": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _u32, _u32,
+                                         _u32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -361,15 +363,18 @@ class State:
 
 
 def nn_gemm(ctx, a_bits, b_bits, c, m, n, k, trans_a=False, trans_b=False, accumulate=False,
-            bias=None):
+            bias=None, noalloc=False):
     """The network's bf16 GEMM on host arrays (bf16 as uint16 bit patterns);
-    returns the float32 result (c updated in place)."""
+    returns the float32 result (c updated in place). `noalloc` stages the
+    operands in the library's static device workspace (clsph_nn_gemm_noalloc)
+    instead of device allocations."""
     a_bits = np.ascontiguousarray(a_bits, dtype=np.uint16)
     b_bits = np.ascontiguousarray(b_bits, dtype=np.uint16)
     c = np.ascontiguousarray(c, dtype=np.float32)
     bias = None if bias is None else np.ascontiguousarray(bias, dtype=np.float32)
-    ctx.check(lib.clsph_nn_gemm(ctx.h, int(trans_a), int(trans_b), int(accumulate), _ptr(a_bits),
-                                _ptr(b_bits), _ptr(c), _ptr(bias), m, n, k))
+    fn = lib.clsph_nn_gemm_noalloc if noalloc else lib.clsph_nn_gemm
+    ctx.check(fn(ctx.h, int(trans_a), int(trans_b), int(accumulate), _ptr(a_bits), _ptr(b_bits),
+                 _ptr(c), _ptr(bias), m, n, k))
     return c
 
 
