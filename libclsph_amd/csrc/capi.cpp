@@ -33,8 +33,10 @@ void free_state_buffers(clsph_state* st) {
   dfree(st->pos_s), dfree(st->vel_s), dfree(st->vh_s), dfree(st->acc_s);
   dfree(st->rho_s), dfree(st->prs_s), dfree(st->fab);
   dfree(st->nbr), dfree(st->nbr_count);
-  dfree(st->key_a), dfree(st->val_a), dfree(st->key_b), dfree(st->val_b);
-  dfree(st->hist), dfree(st->digit_base), dfree(st->table), dfree(st->slot);
+  dfree(st->key_a), dfree(st->key_t), dfree(st->val_t), dfree(st->key_u), dfree(st->val_u);
+  dfree(st->key_b), dfree(st->val_b);
+  dfree(st->sort_hist), dfree(st->sort_totals), dfree(st->sort_runs), dfree(st->sort_runs_ctl);
+  dfree(st->table), dfree(st->slot);
   dfree(st->ids), dfree(st->ids_tmp);
   dfree(st->aabb_partial);
   dfree(st->face_normals), dfree(st->vertices), dfree(st->indices);
@@ -96,6 +98,7 @@ int upload_params(clsph_state* st) {
   b.n_scenes = st->n_scenes;
   b.n = st->n;
   b.table_cap = st->table_cap;
+  b.sort_passes = st->sort_passes;
   HIPCK(st->ctx, hipMemcpyAsync(st->dps, st->host_dps.data(), sizeof(DevParams) * st->n_scenes,
                                 hipMemcpyHostToDevice, st->ctx->stream));
   HIPCK(st->ctx, hipMemcpyAsync(st->bp, &b, sizeof(b), hipMemcpyHostToDevice, st->ctx->stream));
@@ -251,8 +254,6 @@ void enqueue_step(clsph_state* st) {
   launch_grid_params(st);
   launch_hash(st);
   launch_sort(st);
-  launch_permute(st);
-  launch_cell_table(st);
   launch_density(st);
   launch_forces(st);
   launch_advect(st);
@@ -271,8 +272,6 @@ int run_stage(clsph_state* st, int stage) {
       // sorting also builds the tables and the canonical slots, so that a
       // slot is always current once particles have been reordered
       launch_sort(st);
-      launch_permute(st);
-      launch_cell_table(st);
       st->fields_canonical = false;
       break;
     case kStageTable:
@@ -328,12 +327,14 @@ int alloc_state(clsph_ctx* ctx, uint32_t n_scenes, const uint32_t* caps, bool fi
   }
   st->n_cap = (uint32_t)total;
   st->n = fixed ? (uint32_t)total : 0;
-  uint64_t cells = table_cap ? table_cap : std::max<uint64_t>(1u << 20, 8ull * total);
+  uint64_t cells = table_cap ? table_cap : std::max<uint64_t>(1u << 20, 4ull * total);
   if (cells > (1ull << 28)) {
     delete st;
     return fail(ctx, CLSPH_EINVAL, "table_cap above 2^28 cells is not supported");
   }
   st->table_cap = (uint32_t)(8ull * cells);
+  // LSD passes of at most 11 bits over the widest cell id the table admits
+  st->sort_passes = bitlen((uint32_t)(cells - 1)) <= 2u * kMaxDigitBits ? 2u : 3u;
   uint32_t n_cap = st->n_cap;
   st->n_tiles_cap = sort_tiles_for(n_cap ? n_cap : 1);
   int rc = CLSPH_OK;
@@ -351,11 +352,16 @@ int alloc_state(clsph_ctx* ctx, uint32_t n_scenes, const uint32_t* caps, bool fi
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->prs_s, n_cap);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->nbr, (size_t)(kMaxNeighbours / 4) * n_cap);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->nbr_count, n_cap);
-  uint32_t** u4[] = {&st->key_a, &st->val_a, &st->key_b, &st->val_b, &st->slot};
+  uint32_t** u4[] = {&st->key_a, &st->key_t, &st->val_t, &st->key_u, &st->val_u,
+                     &st->key_b, &st->val_b, &st->slot};
   for (auto p : u4)
     if (rc == CLSPH_OK) rc = dalloc(ctx, p, n_cap);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->hist, (size_t)kMaxBins * st->n_tiles_cap);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->digit_base, (size_t)kMaxBins);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->sort_hist, (size_t)st->n_tiles_cap * kMaxBins);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->sort_totals, (size_t)kMaxBins);
+  st->sort_runs_cap = sort_runs_cap_for(st->table_cap);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->sort_runs, (size_t)st->sort_runs_cap);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->sort_runs_ctl, 2);
+  if (rc == CLSPH_OK) HIPCK(ctx, hipMemset(st->sort_runs_ctl, 0, 2 * sizeof(uint32_t)));
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->table, (size_t)st->table_cap + 1);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->aabb_partial, (size_t)(blocks64 + 1) * 6);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->dps, n_scenes);

@@ -21,9 +21,9 @@ enum : uint32_t {
 // groups each cell's particles by octant; the reference's stable order by
 // cell id is recovered from it per particle. The scenes of a batch occupy
 // consecutive key ranges, so one sort orders every scene.
-constexpr int kSortPasses = 3;       // LSD passes; digit width adapts to the key width
-constexpr int kMaxDigitBits = 11;    // 3 x 11 bits covers any 32-bit key
+constexpr int kMaxDigitBits = 11;   // LSD digits of the sort: at most 2048 bins
 constexpr int kMaxBins = 1 << kMaxDigitBits;
+constexpr int kMaxSortPasses = 3;   // 3 x 11 bits cover any cell id
 // Neighbour-list slots per particle; a particle with more neighbours is
 // handled by forces with a cell walk instead.
 constexpr uint32_t kMaxNeighbours = 128;
@@ -66,7 +66,8 @@ struct BatchParams {
   uint32_t key_total;    // half-cell keys of all scenes: 8 * sum(grid_cell_count + 1)
   uint32_t table_cap;    // half-cell table capacity in keys
   uint32_t key_bits;     // bitlen(key_total)
-  uint32_t digit_bits;   // per LSD pass
+  uint32_t sort_passes;  // LSD passes over the cell-id bits (set by the host)
+  uint32_t digit_bits;   // bits per pass: ceil((key_bits - 3) / sort_passes), at most 11
   uint32_t err;
 };
 

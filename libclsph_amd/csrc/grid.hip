@@ -1,6 +1,6 @@
-// Grid stages: bounding box, grid parameters, cell-id hashing, state
-// permutation, cell tables and the reference's stable order, for every scene
-// of a batch at once.
+// Grid stages: bounding box, grid parameters and cell-id hashing, for every
+// scene of a batch at once (the sort, the cell table and the reference's
+// stable order are in sort.hip).
 //
 // The bounding box / grid arithmetic must reproduce the reference's host FP32
 // sequence bit for bit (sph_simulation.cpp:201-252), and the cell id the
@@ -110,8 +110,11 @@ __global__ void k_batch_keys(DevParams* __restrict__ dps, BatchParams* __restric
   bp->key_total = (uint32_t)base;
   uint32_t kb = bitlen((uint32_t)base);
   bp->key_bits = kb;
-  uint32_t db = (kb + kSortPasses - 1) / kSortPasses;
-  bp->digit_bits = db < 1 ? 1 : (db > (uint32_t)kMaxDigitBits ? (uint32_t)kMaxDigitBits : db);
+  // the sort orders by cell id (key >> 3) in bp->sort_passes LSD passes
+  uint32_t cb = kb > 3u ? kb - 3u : 1u;
+  uint32_t passes = bp->sort_passes ? bp->sort_passes : 1u;
+  uint32_t db = (cb + passes - 1u) / passes;
+  bp->digit_bits = db < 1u ? 1u : (db > (uint32_t)kMaxDigitBits ? (uint32_t)kMaxDigitBits : db);
   if (err) atomicOr(&bp->err, err);
 }
 
@@ -121,8 +124,7 @@ __global__ void k_batch_keys(DevParams* __restrict__ dps, BatchParams* __restric
 __global__ __launch_bounds__(kMaxBlock) void k_hash(const float4* __restrict__ pos,
                                                     const BlockDesc* __restrict__ blocks,
                                                     const DevParams* __restrict__ dps,
-                                                    uint32_t* __restrict__ key,
-                                                    uint32_t* __restrict__ val) {
+                                                    uint32_t* __restrict__ key) {
   uint32_t scene;
   uint32_t i = particle_of(blocks, dps, scene);
   if (i == 0xFFFFFFFFu) return;
@@ -137,143 +139,6 @@ __global__ __launch_bounds__(kMaxBlock) void k_hash(const float4* __restrict__ p
     oct |= (((uint32_t)(2.f * u[a])) & 1u) << a;
   }
   key[i] = dp->key_base + ((morton3(c[0], c[1], c[2]) << 3) | oct);
-  val[i] = i;
-}
-
-// Gathers the canonical state into sorted order: s[i] = canonical[perm[i]].
-__global__ __launch_bounds__(256) void k_permute(
-    const uint32_t* __restrict__ perm, const float4* __restrict__ pos,
-    const float4* __restrict__ vel, const float4* __restrict__ vh,
-    float4* __restrict__ pos_s, float4* __restrict__ vel_s,
-    float4* __restrict__ vh_s, uint32_t n) {
-  uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  uint32_t j = perm[i];
-  pos_s[i] = pos[j];
-  vel_s[i] = vel[j];
-  vh_s[i] = vh[j];
-}
-
-// First index in [lo, hi) whose key is >= k (keys sorted ascending).
-__device__ inline uint32_t lower_bound(const uint32_t* __restrict__ keys, uint32_t lo,
-                                       uint32_t hi, uint32_t k) {
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (keys[mid] < k)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
-// Half-cell table: ht[f] = #{keys < f} for f in [0, key_total], one thread
-// per group of 8 keys (a cell of side 2h). The reference's cell table of a
-// scene (sph_simulation.cpp:162-170), T[c] = #{cell ids < c}, is
-// ht[key_base + 8c] - begin.
-__global__ __launch_bounds__(256) void k_cell_table(const uint32_t* __restrict__ keys,
-                                                    uint32_t n,
-                                                    const BatchParams* __restrict__ bp,
-                                                    uint32_t* __restrict__ ht) {
-  uint32_t total = bp->key_total;
-  if (total > bp->table_cap || (bp->err & kErrFineKey)) return;
-  uint32_t cells = total >> 3;
-  for (uint32_t c = blockIdx.x * 256 + threadIdx.x; c < cells; c += gridDim.x * 256) {
-    uint32_t k0 = c << 3;
-    uint32_t start = lower_bound(keys, 0, n, k0);
-    // end of the cell: gallop from start, then bisect
-    uint32_t nxt = k0 + 8u;
-    uint32_t lo = start, step = 64, hi = start;
-    while (hi < n && keys[hi] < nxt) {
-      lo = hi + 1;
-      hi = (n - hi > step) ? hi + step : n;
-      step <<= 1;
-    }
-    uint32_t end = lower_bound(keys, lo, hi, nxt);
-    ht[k0] = start;
-    // the octants' bounds inside [start, end): seven searches stepping
-    // together, so that each step issues up to seven independent loads
-    uint32_t ol[7], oh[7];
-#pragma unroll
-    for (int o = 0; o < 7; ++o) ol[o] = start, oh[o] = end;
-    bool more = start < end;
-    while (more) {
-      more = false;
-#pragma unroll
-      for (int o = 0; o < 7; ++o) {
-        if (ol[o] < oh[o]) {
-          const uint32_t mid = (ol[o] + oh[o]) >> 1;
-          if (keys[mid] < (k0 | (uint32_t)(o + 1)))
-            ol[o] = mid + 1;
-          else
-            oh[o] = mid;
-          more = more || ol[o] < oh[o];
-        }
-      }
-    }
-#pragma unroll
-    for (int o = 0; o < 7; ++o) ht[k0 + 1 + o] = ol[o];
-    if (c == cells - 1) ht[total] = n;
-  }
-}
-
-// Slot of the particle at sorted position k in the reference's order (stable
-// by cell id). Within a cell the half-cell order is octant-major with the
-// canonical (input) order kept inside each octant, so the particle's rank in
-// its cell is its index in its octant plus, for each other octant, the
-// members that came before it in the input: a lower bound of its canonical
-// index in that octant's run. The seven searches step together, so each
-// step issues up to seven independent loads.
-__device__ inline uint32_t slot_by_search(uint32_t k, const uint32_t* __restrict__ keys,
-                                          const uint32_t* __restrict__ perm,
-                                          const uint32_t* __restrict__ ht, uint32_t total) {
-  const uint32_t f = keys[k], c8 = f & ~7u, o = f & 7u;
-  if (f >= total || c8 + 8u > total) return k;  // keys that are not this table's
-  const uint32_t me = perm[k];
-  uint32_t b[9];
-#pragma unroll
-  for (int q = 0; q < 9; ++q) b[q] = ht[c8 + q];
-  uint32_t lo[8], hi[8], base = 0;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    lo[q] = b[q];
-    hi[q] = (uint32_t)q == o ? b[q] : b[q + 1];  // the own octant is counted directly
-    base += b[q];
-  }
-  bool more = true;
-  while (more) {
-    more = false;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      if (lo[q] < hi[q]) {
-        const uint32_t mid = (lo[q] + hi[q]) >> 1;
-        if (perm[mid] < me)
-          lo[q] = mid + 1;
-        else
-          hi[q] = mid;
-        more = more || lo[q] < hi[q];
-      }
-    }
-  }
-  uint32_t rank = k - b[o];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) rank += lo[q];
-  return b[0] + rank - base;
-}
-
-__global__ __launch_bounds__(256) void k_canonical_slot(const uint32_t* __restrict__ keys,
-                                                        const uint32_t* __restrict__ perm,
-                                                        const uint32_t* __restrict__ ht,
-                                                        const BatchParams* __restrict__ bp,
-                                                        uint32_t* __restrict__ slot, uint32_t n) {
-  uint32_t k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= n) return;
-  const uint32_t total = bp->key_total;
-  if (total > bp->table_cap || (bp->err & kErrFineKey)) {
-    slot[k] = k;
-    return;
-  }
-  slot[k] = slot_by_search(k, keys, perm, ht, total);
 }
 
 void launch_aabb(clsph_state* s) {
@@ -291,23 +156,7 @@ void launch_grid_params(clsph_state* s) {
 void launch_hash(clsph_state* s) {
   if (s->n == 0) return;
   hipLaunchKernelGGL(k_hash, dim3(s->nblocks_of(kStageHash)), dim3(s->bs_of(kStageHash)), 0,
-                     s->ctx->stream, s->pos, s->table_of(kStageHash), s->dps, s->key_a,
-                     s->val_a);
-}
-
-void launch_permute(clsph_state* s) {
-  if (s->n == 0) return;
-  hipLaunchKernelGGL(k_permute, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream,
-                     s->val_b, s->pos, s->vel, s->vh, s->pos_s, s->vel_s, s->vh_s, s->n);
-}
-
-void launch_cell_table(clsph_state* s) {
-  if (s->n == 0) return;
-  uint32_t blocks = (uint32_t)s->ctx->cus * 8;
-  hipLaunchKernelGGL(k_cell_table, dim3(blocks), dim3(256), 0, s->ctx->stream, s->key_b, s->n,
-                     s->bp, s->table);
-  hipLaunchKernelGGL(k_canonical_slot, dim3((s->n + 255) / 256), dim3(256), 0, s->ctx->stream,
-                     s->key_b, s->val_b, s->table, s->bp, s->slot, s->n);
+                     s->ctx->stream, s->pos, s->table_of(kStageHash), s->dps, s->key_a);
 }
 
 }  // namespace clsph

@@ -1170,8 +1170,6 @@ void launch_target_mean(clsph_nn* m, clsph_state* learner, clsph_state* teacher,
     launch_grid_params(teacher);
     launch_hash(teacher);
     launch_sort(teacher);
-    launch_permute(teacher);
-    launch_cell_table(teacher);
     launch_density(teacher);
     launch_forces(teacher);
     hipLaunchKernelGGL(k_accum_acc, grid_for(n), dim3(256), 0, s, teacher->acc_s, teacher->val_b,

@@ -97,8 +97,6 @@ void enqueue_density_stages(clsph_state* st) {
   launch_grid_params(st);
   launch_hash(st);
   launch_sort(st);
-  launch_permute(st);
-  launch_cell_table(st);
   launch_density(st);
   st->acc_learned = false;
   st->fields_canonical = false;
@@ -124,8 +122,6 @@ int ensure_learned_graph(clsph_state* st, clsph_nn* m) {
   launch_grid_params(st);
   launch_hash(st);
   launch_sort(st);
-  launch_permute(st);
-  launch_cell_table(st);
   launch_density(st, (float2*)m->nu);
   launch_forces_overflow(st);
   nn::forward_fused(m, st, st->acc_s, nullptr, true, nullptr, true);

@@ -55,14 +55,26 @@ struct clsph_state {
   uint4* nbr = nullptr;           // neighbour lists, 4 per entry: nbr[q * n + i]
   uint32_t* nbr_count = nullptr;  // neighbours per particle
 
-  // Radix sort buffers: keys/values ping-pong; after kSortPasses (odd) passes
-  // the sorted keys and permutation sit in key_b / val_b.
+  // Sort buffers: half-cell keys in canonical order (key_a); the LSD passes'
+  // outputs (key_t / val_t after odd passes, key_u / val_u after even ones,
+  // values = canonical indices); the sorted keys and permutation (key_b /
+  // val_b: sorted position -> canonical index).
   uint32_t* key_a = nullptr;
-  uint32_t* val_a = nullptr;
+  uint32_t* key_t = nullptr;
+  uint32_t* val_t = nullptr;
+  uint32_t* key_u = nullptr;
+  uint32_t* val_u = nullptr;
   uint32_t* key_b = nullptr;
   uint32_t* val_b = nullptr;
-  uint32_t* hist = nullptr;        // per-tile digit counts, digit-major
-  uint32_t* digit_base = nullptr;  // digit totals
+  uint32_t sort_passes = 2;           // LSD passes, from the table capacity
+  uint32_t* sort_hist = nullptr;      // per-tile digit counts, then offsets (tiles x kMaxBins)
+  uint32_t* sort_totals = nullptr;    // digit totals of the current pass (kMaxBins)
+  // long runs of empty keys of the half-cell table, listed by the cell pass
+  // and written by the fill pass ({first key, end key, value, 0}); the list's
+  // length (emptied before each cell pass) and a spare word
+  uint4* sort_runs = nullptr;
+  uint32_t* sort_runs_ctl = nullptr;
+  uint32_t sort_runs_cap = 0;
   uint32_t n_tiles_cap = 0;
 
   // Half-cell table (table_cap + 1 entries): table[f] = #{keys < f}; the
@@ -153,15 +165,18 @@ namespace clsph {
 void launch_aabb(clsph_state* s);
 void launch_grid_params(clsph_state* s);
 void launch_hash(clsph_state* s);
-void launch_permute(clsph_state* s);
-void launch_cell_table(clsph_state* s);
 void launch_track_ids(clsph_state* s);
 // Particle records (reference layout, canonical order) of a single-scene
 // state into out (device, n records), as the host download assembles them.
 void launch_pack_aos(clsph_state* s, clsph_particle* out);
 // sort.hip
+// The sort: sorted keys and permutation, the half-cell table, the canonical
+// slots and the state gathered into sorted order.
 void launch_sort(clsph_state* s);
+// Its last pass alone (table, slots and gather again from the same grouping).
+void launch_cell_table(clsph_state* s);
 uint32_t sort_tiles_for(uint32_t n);
+uint32_t sort_runs_cap_for(uint32_t table_cap);
 // sph.hip
 // With `nu`, also the learned surrogate's node inputs of every particle.
 void launch_density(clsph_state* s, float2* nu = nullptr);
