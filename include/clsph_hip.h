@@ -52,7 +52,7 @@ int clsph_sync(clsph_ctx* ctx);
 /* ---- simulation state: replaces the front/back/sort_count cl::Buffers
  * (sph_simulation.cpp:373-380). A state holds up to n_cap particles of one
  * scene. `table_cap` is the cell-table capacity in cells of side 2h
- * (0 = max(2^20, 8 n_cap)); a substep whose grid has more cells fails with
+ * (0 = max(2^20, 4 n_cap)); a substep whose grid has more cells fails with
  * CLSPH_ETABLE. */
 int clsph_state_alloc(clsph_ctx* ctx, uint32_t n_cap, uint32_t table_cap,
                       clsph_state** out);
@@ -116,7 +116,7 @@ int clsph_step(clsph_state* st, uint32_t nsteps);
  * launch over all scenes. Scene s holds exactly counts[s] particles; each
  * scene has its own parameters, mesh, grid and cell table, and its results
  * are those of a state holding that scene alone. `table_cap` counts cells of
- * all scenes together (0 = max(2^20, 8 * sum(counts))). The single-scene
+ * all scenes together (0 = max(2^20, 4 * sum(counts))). The single-scene
  * calls above act on scene 0. */
 int clsph_batch_alloc(clsph_ctx* ctx, uint32_t n_scenes, const uint32_t* counts,
                       uint32_t table_cap, clsph_state** out);
