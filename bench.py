@@ -57,18 +57,39 @@ def stage_bytes(n, cells):
 STAGE_KERNEL = {"hash": "clsph::k_hash", "density": "clsph::k_density",
                 "forces": "clsph::k_forces", "advect": "clsph::k_advect"}
 
+# VALU issue: a wave64 VALU instruction occupies one of the 1,024 SIMDs (256
+# CUs x 4) for 4 cycles at up to 2.4 GHz.
+SIMDS = 1024
+CLOCK_HZ = 2.4e9
 
-def measured_traffic(stage, workload):
-    """HBM bytes per launch of the stage's kernel from the committed rocprof
-    counter summary (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE runs,
-    scaled by a copy kernel of known size), or None."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
-    if workload != "dam_break" or not os.path.exists(path):
-        return None
-    with open(path) as f:
-        t = json.load(f)
-    v = t["kernels"].get(STAGE_KERNEL.get(stage, ""))
-    return float(v) if v is not None else None
+
+def measured_counters(stage, workload, n):
+    """HBM bytes and VALU instructions per launch of the stage's kernel from
+    profiles/counters.json (rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a
+    copy kernel of known size, and SQ_INSTS_VALU), when those counters were
+    taken on the sources being run (libclsph_amd.counters.source_stamp) and
+    this workload; else {}."""
+    from libclsph_amd import counters
+    data = counters.load()
+    if data is None or workload != "dam_break" or data.get("workload") != f"dam_break {n}":
+        return {}
+    return data["kernels"].get(STAGE_KERNEL.get(stage, ""), {})
+
+
+def roofline(dom, achieved, ms, workload, n):
+    """The dominant kernel against the HBM peak (algorithmic bytes over its
+    measured duration) and, from the committed counters of the same sources,
+    its measured HBM traffic and its VALU issue fraction (VALU instructions x
+    4 cycles over the SIMDs' cycles in its duration): the bound it meets."""
+    c = measured_counters(dom, workload, n)
+    out = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": c.get("traffic")}
+    if c.get("traffic"):
+        out["traffic_frac"] = c["traffic"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+    if c.get("valu"):
+        out["valu_issue_frac"] = c["valu"] * 4.0 / (SIMDS * CLOCK_HZ * ms * 1e-3)
+        out["limiter"] = "valu" if out["valu_issue_frac"] > out.get("traffic_frac", 0.0) else "hbm"
+    return out
 
 
 def cpu_baseline(scene, label, budget_s=20.0, max_steps=3):
@@ -240,15 +261,7 @@ def main():
             "density_pairs_per_s": pairs / (prof["density"] * 1e-3),
             "forces_pairs_per_s": pairs / (prof["forces"] * 1e-3),
         },
-        "roofline": {
-            "kernel": dom,
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom, args.workload) if n == 1048576 else None,
-        },
+        "roofline": roofline(dom, achieved, prof[dom], args.workload, n),
     }
     if learned is not None:
         out["learned"] = learned
@@ -269,6 +282,10 @@ def main():
 # Matrix work of one surrogate evaluation over the used inputs: the edge
 # layer (27 inputs -> 32) per edge and the node MLP (38 -> 64 -> 64 -> 3)
 # per particle, 2 flops per multiply-add.
+NN_SHAPE = ("1 round of message passing over the neighbours within h (at most 128): edge "
+            "layer 27 -> 32 (ReLU), summed per particle; node MLP 38 -> 64 -> 64 -> 3 (ReLU); "
+            "output corrects the forces stage's acceleration; bf16 weights and activations, "
+            "fp32 accumulation")
 NN_EDGE_FLOP = 2.0 * 27 * 32
 NN_NODE_FLOP = 2.0 * (38 * 64 + 64 * 64 + 64 * 3)
 MFMA_BF16_PEAK_TFLOPS = 2500.0
@@ -319,8 +336,13 @@ def bench_learned(ctx, plain, scene, plain_ms_per_step, macro_steps, judge_every
     # and with it
     learner.copy_from(start)
     r = lrn.run_learned(ctx, learner, teacher, net, macro_steps, dt, judge_every, tol)
+    # the plain method at the same 4 dt step (the method without the network)
     big.copy_from(start)
+    ctx.sync()
+    t0 = time.perf_counter()
     big.step(macro_steps)
+    ctx.sync()
+    big_wall = time.perf_counter() - t0
     for s in (plain, learner, big):
         lrn.density_stages(s)
     err_max, err_mean = net.compare_density(learner, plain)
@@ -340,12 +362,18 @@ def bench_learned(ctx, plain, scene, plain_ms_per_step, macro_steps, judge_every
     return {
         "interval": {"substeps": macro_steps * lrn.MACRO, "sim_s": sim_s,
                      "plain_ms_per_substep": plain_wall / (macro_steps * lrn.MACRO) * 1e3},
+        "network": NN_SHAPE,
+        "network_parameters": int(native.lib.clsph_nn_param_count()),
         "sim_s_per_wall_s": {"plain": plain_rate, "learned": r.sim_s / r.wall_s,
                              "ratio": (r.sim_s / r.wall_s) / plain_rate,
                              "learned_unjudged": bare.sim_s / bare.wall_s,
-                             "ratio_unjudged": (bare.sim_s / bare.wall_s) / plain_rate},
+                             "ratio_unjudged": (bare.sim_s / bare.wall_s) / plain_rate,
+                             "plain_4dt": sim_s / big_wall,
+                             "ratio_plain_4dt": (sim_s / big_wall) / plain_rate,
+                             "learned_over_plain_4dt": (r.sim_s / r.wall_s) / (sim_s / big_wall)},
         "macro_step_dt": lrn.MACRO * dt,
         "macro_steps": r.macro_steps,
+        "unjudged_fraction": 1.0 - r.judged / max(1, r.macro_steps),
         "ms_per_macro_step": {"unjudged": bare.wall_s / macro_steps * 1e3,
                               "judged": r.wall_s / macro_steps * 1e3},
         "judge_every": judge_every,

@@ -78,3 +78,34 @@ def test_batch_grid_matches_oracle_per_scene(ctx):
         np.testing.assert_array_equal(g["min_point"][:3], p["min_point"][:3])
         assert int(g["grid_cell_count"]) == int(p["grid_cell_count"])
     b.close()
+
+
+def test_full_batch_64_scenes_equal_scenes_alone(ctx):
+    """The benchmark's batch: 64 dam breaks of 4,096-262,144 particles
+    (log-uniform, seeded), viscosity 1e-3-5e-2, with the autotuner's
+    work-group sizes. After 5 substeps every scene is bit-identical to the
+    same scene run alone; before them every scene's grid fields equal the
+    oracle's."""
+    scs = scenes.dam_break_batch(64)
+    counts = [s.n for s in scs]
+    print(f"case 64-scene batch, {sum(counts)} particles, {min(counts)}-{max(counts)}",
+          flush=True)
+    assert len(scs) == 64 and min(counts) >= 4096 and max(counts) <= 262144
+    b = autotune.make_batch(ctx, scs)
+    b.set_block_sizes(autotune.sizes_for(counts))
+    b.stage("grid")
+    for i, sc in enumerate(scs):
+        p = sc.params.copy()
+        assert oracle.grid_params(sc.particles, p) == 0
+        g = b.grid_scene(i)
+        for f in ("min_point", "max_point"):
+            np.testing.assert_array_equal(g[f][:3], p[f][:3], err_msg=f"scene {i} {f}")
+        for f in ("grid_size_x", "grid_size_y", "grid_size_z", "grid_cell_count"):
+            assert int(g[f]) == int(p[f]), (i, f)
+    b.step(5)
+    for i, sc in enumerate(scs):
+        got = b.download_scene(i)
+        want = run_alone(ctx, sc, 5)
+        for f in FIELDS:
+            np.testing.assert_array_equal(got[f], want[f], err_msg=f"scene {i} {f}")
+    b.close()

@@ -6,7 +6,8 @@ On a dam-break variant the training never saw, from one common state:
   (c) the learned method at 4 dt            (the plain substep plus the network's correction)
 Densities of the same particles are compared at the matched time. The
 learned run must stay finite with its density error against (a) inside the
-judge's bound, and the test prints what it gains over (b). On states of a
+judge's bound, and it must beat (b) in mean and max density error against (a)
+(an expected failure while the shipped network does not). On states of a
 held-out variant the network's squared error against its training target
 (the mean forces-stage acceleration over the next 4 substeps) must be below
 that of the method without it. A separate case compares both GPU runs with
@@ -51,7 +52,11 @@ def density_by_id(st):
     return out
 
 
-def test_learned_gain_over_plain_at_large_step(ctx):
+@pytest.fixture(scope="module")
+def large_step_errors(ctx):
+    """Density errors against the plain run at dt, on a held-out slab after
+    100 substeps' worth of time, of the plain method at 4 dt and of the
+    learned method at 4 dt (unjudged)."""
     sc = learned.slab_variant(100003)
     dt = float(sc.params["time_delta"])
     rho0 = float(sc.params["fluid_density"])
@@ -82,12 +87,32 @@ def test_learned_gain_over_plain_at_large_step(ctx):
     mx, mean = net.compare_density(runs["learned_4dt"], runs["plain_dt"])
     assert abs(mx - err["learned_4dt"].max()) <= 1e-6
     assert abs(mean - err["learned_4dt"].mean()) <= 1e-6
-    # Over 100 substeps the maximum is set by a few splashing particles whose
-    # paths diverge; the mean measures the field. Unjudged, the learned run
-    # stays within 1 % of the plain density field on average.
-    assert err["learned_4dt"].mean() <= 0.01
-    print(f"mean density error, plain at 4 dt / learned at 4 dt: "
-          f"{err['plain_4dt'].mean() / max(err['learned_4dt'].mean(), 1e-12):.2f}x", flush=True)
+    net.close()
+    for st in list(runs.values()) + [base]:
+        st.close()
+    return err
+
+
+def test_learned_run_stays_within_the_judge_bound(large_step_errors):
+    """Unjudged over 100 substeps' worth of time, the learned run stays
+    within 1 % of the plain density field on average (the maximum is set by a
+    few splashing particles whose paths diverge)."""
+    assert large_step_errors["learned_4dt"].mean() <= 0.01
+
+
+@pytest.mark.xfail(strict=False, reason=(
+    "the shipped network does not beat the plain step at 4 dt on held-out slabs: its density "
+    "error against the plain run at dt is larger in mean and in max"))
+def test_learned_beats_plain_at_the_same_large_step(large_step_errors):
+    """What the network has to gain: at the same 4 dt step, the learned
+    method's density error against the plain run at dt must be below that of
+    the plain method without the network, in mean and in max."""
+    err = large_step_errors
+    print(f"mean error plain 4 dt / learned 4 dt: "
+          f"{err['plain_4dt'].mean() / max(err['learned_4dt'].mean(), 1e-12):.2f}x, "
+          f"max: {err['plain_4dt'].max() / max(err['learned_4dt'].max(), 1e-12):.2f}x", flush=True)
+    assert err["learned_4dt"].mean() < err["plain_4dt"].mean()
+    assert err["learned_4dt"].max() < err["plain_4dt"].max()
 
 
 def held_out_loss(ctx, net, seed=100007, samples=8, gap=40, warm=300):
@@ -144,6 +169,13 @@ def test_judged_run_counts_fallbacks(ctx):
           f"{np.round(r.judge_err, 5).tolist()}", flush=True)
     assert r.judged == 8 and 0 <= r.fallbacks <= r.judged
     assert all(np.isfinite(r.judge_err))
+    # every macro step judged: the count of fallbacks over 16 macro steps
+    learner.copy_from(base)
+    r1 = learned.run_learned(ctx, learner, teacher, net, 16, dt, judge_every=1, tol=0.01)
+    print(f"judged every step: {r1.judged} judged, {r1.fallbacks} fallbacks, worst judged "
+          f"error {max(r1.judge_err):.4f}", flush=True)
+    assert r1.judged == 16 and 0 <= r1.fallbacks <= 16
+    assert all(np.isfinite(r1.judge_err))
     # a tolerance of 0 forces a fallback at every judged step: the result is
     # then the plain trajectory at the judged steps
     learner.copy_from(base)
