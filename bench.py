@@ -289,6 +289,51 @@ NN_SHAPE = ("1 round of message passing over the neighbours within h (at most 12
 NN_EDGE_FLOP = 2.0 * 27 * 32
 NN_NODE_FLOP = 2.0 * (38 * 64 + 64 * 64 + 64 * 3)
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+# Two-round network (clsph_gnn_*): per edge and round, 16 -> 128 -> 128;
+# per particle, the two projections of H for round 2 and the third edge
+# layer of each round (128 x 128 each), node MLP 1 (144 -> 128 -> 128 -> 128)
+# and node MLP 2 (256 -> 128 -> 128 -> 32).
+GNN_SHAPE = ("2 rounds of message passing over the neighbours within h (at most 128): edge "
+             "MLP 16 -> 128 -> 128 (SiLU), summed, then 128 -> 128; node MLPs 144 -> 128 -> "
+             "128 -> 128 and 256 -> 128 -> 128 -> 32 (SiLU); bf16 weights and activations, "
+             "fp32 accumulation")
+GNN_EDGE_FLOP = 2 * 2.0 * (16 * 128 + 128 * 128)
+GNN_NODE_FLOP = 2.0 * (4 * 128 * 128 + 144 * 128 + 2 * 128 * 128 + 256 * 128 + 128 * 128
+                       + 128 * 32)
+
+
+def gnn_params(count, seed=0):
+    """Seeded weights of the two-round network for timing (values do not
+    change its work): U(-1, 1) / sqrt(128) for every entry."""
+    rng = np.random.default_rng(seed)
+    return (rng.uniform(-1.0, 1.0, count) / np.sqrt(128.0)).astype(np.float32)
+
+
+def bench_gnn(ctx, state, plain_ms_per_step):
+    """Fused inference time of the two-round network on a state whose last
+    stage was density, its MFMA rate, and its time at the bf16 peak beside
+    the plain method's 4 substeps that one learned macro step replaces."""
+    from libclsph_amd import native
+    net = native.TwoRoundNetwork(ctx, state.n, e_cap=1024)
+    net.set_params(gnn_params(net.count))
+    ms, edges = net.profile(state, reps=5)
+    net.close()
+    total = sum(ms)
+    flop = edges * GNN_EDGE_FLOP + state.n * GNN_NODE_FLOP
+    tflops = flop / (total * 1e-3) / 1e12
+    return {
+        "network": GNN_SHAPE,
+        "network_parameters": int(native.lib.clsph_gnn_param_count()),
+        "ms": total,
+        "ms_parts": {"edge_round_1": ms[0], "node_round_1": ms[1], "edge_round_2": ms[2],
+                     "node_round_2": ms[3]},
+        "edges": edges,
+        "tflop_per_evaluation": flop / 1e12,
+        "ms_at_mfma_peak": flop / (MFMA_BF16_PEAK_TFLOPS * 1e12) * 1e3,
+        "plain_ms_per_4_substeps": 4.0 * plain_ms_per_step,
+        "mfma": {"unit": "TFLOP/s", "peak": MFMA_BF16_PEAK_TFLOPS, "achieved": tflops,
+                 "frac": tflops / MFMA_BF16_PEAK_TFLOPS},
+    }
 
 
 def bench_learned(ctx, plain, scene, plain_ms_per_step, macro_steps, judge_every=16, tol=0.01):
@@ -324,6 +369,9 @@ def bench_learned(ctx, plain, scene, plain_ms_per_step, macro_steps, judge_every
     learner.copy_from(start)
     lrn.forces_stages(learner)
     net_ms, edges = net.profile(learner, reps=5)
+    learner.copy_from(start)
+    lrn.density_stages(learner)
+    gnn = bench_gnn(ctx, learner, plain_ms_per_step)
     # plain substeps over the interval
     ctx.sync()
     t0 = time.perf_counter()
@@ -389,6 +437,7 @@ def bench_learned(ctx, plain, scene, plain_ms_per_step, macro_steps, judge_every
         "network_kernel_mfma": {
             "unit": "TFLOP/s", "peak": MFMA_BF16_PEAK_TFLOPS, "achieved": tflops,
             "frac": tflops / MFMA_BF16_PEAK_TFLOPS},
+        "two_round_network": gnn,
     }
 
 

@@ -305,6 +305,51 @@ int clsph_nn_gemm_noalloc(clsph_ctx* ctx, int trans_a, int trans_b, int accumula
                           const uint16_t* a, const uint16_t* b, float* c, const float* bias,
                           uint32_t m, uint32_t n, uint32_t k);
 
+/* ---- two-round network ----
+ * No counterpart in the reference, whose only acceleration model is the
+ * `forces` kernel (kernels/sph.cl:42-62). The full-size network a learned
+ * force surrogate can use: two rounds of message passing over the neighbour
+ * graph of the density stage (r < h, at most 128 neighbours per particle),
+ * three 128-wide SiLU layers per edge and per node network, output
+ * a = g + 10 out[0:3] (m/s^2). Its products run on bf16 matrix cores with
+ * float32 accumulation. Parameters are one float32 vector of
+ * clsph_gnn_param_count() values (row-major inputs x outputs matrices and
+ * biases in a fixed order). Single-scene states only. */
+typedef struct clsph_gnn clsph_gnn;
+uint32_t clsph_gnn_param_count(void);
+/* Workspace for states of up to n_cap particles and graphs of up to e_cap
+ * edges (0: 64 n_cap); parameters start at zero. */
+int clsph_gnn_create(clsph_ctx* ctx, uint32_t n_cap, uint32_t e_cap, clsph_gnn** out);
+int clsph_gnn_free(clsph_gnn* nn);
+/* Sets the parameters (and clears the optimiser moments) / reads them. */
+int clsph_gnn_set_params(clsph_gnn* nn, const float* params, uint32_t count);
+int clsph_gnn_get_params(clsph_gnn* nn, float* params, uint32_t count);
+/* Evaluates the network on a state whose last stage was density: builds the
+ * graph, writes the accelerations where the forces stage would (the
+ * following advection uses them) and the edge count to *edges (optional). */
+int clsph_gnn_forward(clsph_gnn* nn, clsph_state* st, uint32_t* edges);
+/* The same evaluation with the fused inference kernels: no per-edge
+ * storage (any edge count), no host synchronisation, no gradients. */
+int clsph_gnn_infer(clsph_gnn* nn, clsph_state* st);
+/* Runs clsph_gnn_infer `reps` times with HIP events on the context stream;
+ * ms[0..3] = average milliseconds of edge round 1, the node part of round 1,
+ * edge round 2 and the node part of round 2; *edges (optional) = the edges
+ * evaluated per pass. */
+int clsph_gnn_profile(clsph_gnn* nn, clsph_state* st, uint32_t reps, float* ms, uint64_t* edges);
+/* The graph of the last evaluation (dst-major CSR: off[n + 1], src[e]) and
+ * the raw network output (n x 32 floats; columns 0..2 are (a - g) / 10). */
+int clsph_gnn_read_graph(clsph_gnn* nn, uint32_t* off, uint32_t* src, uint32_t n, uint32_t e);
+int clsph_gnn_read_output(clsph_gnn* nn, float* out, uint32_t n);
+/* Training target (n x 3, in the output's units). */
+int clsph_gnn_set_target(clsph_gnn* nn, const float* target, uint32_t n);
+int clsph_gnn_read_target(clsph_gnn* nn, float* target, uint32_t n);
+/* Mean squared error of the last clsph_gnn_forward against the target and
+ * its gradient with respect to the parameters (*loss optional), then one
+ * Adam step (bias-corrected). */
+int clsph_gnn_backward(clsph_gnn* nn, float* loss);
+int clsph_gnn_read_grad(clsph_gnn* nn, float* grad, uint32_t count);
+int clsph_gnn_adam(clsph_gnn* nn, float lr, float beta1, float beta2, float eps);
+
 /* ---- scene builders (host only; no device needed) ---- */
 /* Dam-break block: a cubic lattice of `n` particles at `spacing` with
  * round(extent/spacing) sites along x and z, stacked layer by layer from

@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libclsph_hip.so")
 
-SOURCES = ["grid.hip", "sort.hip", "sph.hip", "advect.hip", "nn.hip", "capi.cpp", "nn_capi.cpp"]
+SOURCES = ["grid.hip", "sort.hip", "sph.hip", "advect.hip", "nn.hip", "gnn.hip", "capi.cpp",
+           "nn_capi.cpp", "gnn_capi.cpp"]
 API = os.path.join(HERE, "api")
 API_SOURCES = ["sph_simulation.cpp", "scene.cpp", "json.cpp",
                "file_save_delegates/houdini_file_saver.cpp", "util/cl_boilerplate.cpp"]

@@ -115,6 +115,22 @@ def _load():
         "clsph_nn_gemm": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _u32, _u32, _u32]),
         "clsph_nn_gemm_noalloc": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _u32, _u32,
                                          _u32]),
+        "clsph_gnn_param_count": (_u32, []),
+        "clsph_gnn_create": (_i32, [_vp, _u32, _u32, ctypes.POINTER(_vp)]),
+        "clsph_gnn_free": (_i32, [_vp]),
+        "clsph_gnn_set_params": (_i32, [_vp, _vp, _u32]),
+        "clsph_gnn_get_params": (_i32, [_vp, _vp, _u32]),
+        "clsph_gnn_forward": (_i32, [_vp, _vp, _vp]),
+        "clsph_gnn_infer": (_i32, [_vp, _vp]),
+        "clsph_gnn_profile": (_i32, [_vp, _vp, _u32, _vp, _vp]),
+        "clsph_gnn_read_graph": (_i32, [_vp, _vp, _vp, _u32, _u32]),
+        "clsph_gnn_read_output": (_i32, [_vp, _vp, _u32]),
+        "clsph_gnn_set_target": (_i32, [_vp, _vp, _u32]),
+        "clsph_gnn_read_target": (_i32, [_vp, _vp, _u32]),
+        "clsph_gnn_backward": (_i32, [_vp, _vp]),
+        "clsph_gnn_read_grad": (_i32, [_vp, _vp, _u32]),
+        "clsph_gnn_adam": (_i32, [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                  ctypes.c_float]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -499,6 +515,100 @@ class Network:
         mn = np.zeros(1, dtype=np.float32)
         self.ctx.check(lib.clsph_nn_compare_density(self.h, a.h, b.h, _ptr(mx), _ptr(mn)))
         return float(mx[0]), float(mn[0])
+
+
+class TwoRoundNetwork:
+    """The two-round message-passing network's device workspace and
+    parameters (clsph_gnn_*)."""
+
+    def __init__(self, ctx, n_cap, e_cap=0):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(lib.clsph_gnn_create(ctx.h, n_cap, e_cap, ctypes.byref(h)))
+        self.h = h
+        self.count = int(lib.clsph_gnn_param_count())
+        self.n = 0
+        self.e = 0
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib.clsph_gnn_free(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, flat):
+        flat = np.ascontiguousarray(flat, dtype=np.float32)
+        assert flat.shape == (self.count,)
+        self.ctx.check(lib.clsph_gnn_set_params(self.h, _ptr(flat), self.count))
+
+    def params(self):
+        out = np.zeros(self.count, dtype=np.float32)
+        self.ctx.check(lib.clsph_gnn_get_params(self.h, _ptr(out), self.count))
+        return out
+
+    def forward(self, state):
+        """Training evaluation on a state whose last stage was density: its
+        accelerations become the network's; returns the edge count."""
+        e = np.zeros(1, dtype=np.uint32)
+        self.ctx.check(lib.clsph_gnn_forward(self.h, state.h, _ptr(e)))
+        self.n, self.e = state.n, int(e[0])
+        return self.e
+
+    def infer(self, state):
+        """Fused inference on a state whose last stage was density."""
+        self.ctx.check(lib.clsph_gnn_infer(self.h, state.h))
+        self.n = state.n
+
+    def profile(self, state, reps=5):
+        """Average ms of edge round 1, node round 1, edge round 2 and node
+        round 2 of the fused evaluation over `reps` runs, and the edges per
+        pass."""
+        ms = np.zeros(4, dtype=np.float32)
+        e = np.zeros(1, dtype=np.uint64)
+        self.ctx.check(lib.clsph_gnn_profile(self.h, state.h, reps, _ptr(ms), _ptr(e)))
+        self.n = state.n
+        return [float(x) for x in ms], int(e[0])
+
+    def graph(self):
+        off = np.zeros(self.n + 1, dtype=np.uint32)
+        src = np.zeros(self.e, dtype=np.uint32)
+        self.ctx.check(lib.clsph_gnn_read_graph(self.h, _ptr(off), _ptr(src), self.n, self.e))
+        return off, src
+
+    def output(self):
+        out = np.zeros((self.n, 32), dtype=np.float32)
+        self.ctx.check(lib.clsph_gnn_read_output(self.h, _ptr(out), self.n))
+        return out
+
+    def set_target(self, target):
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        assert t.shape == (self.n, 3)
+        self.ctx.check(lib.clsph_gnn_set_target(self.h, _ptr(t), self.n))
+
+    def target(self, n=None):
+        n = self.n if n is None else n
+        t = np.zeros((n, 3), dtype=np.float32)
+        self.ctx.check(lib.clsph_gnn_read_target(self.h, _ptr(t), n))
+        return t
+
+    def backward(self, read_loss=True):
+        loss = np.zeros(1, dtype=np.float32)
+        self.ctx.check(lib.clsph_gnn_backward(self.h, _ptr(loss) if read_loss else None))
+        return float(loss[0]) if read_loss else None
+
+    def grad(self):
+        out = np.zeros(self.count, dtype=np.float32)
+        self.ctx.check(lib.clsph_gnn_read_grad(self.h, _ptr(out), self.count))
+        return out
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.ctx.check(lib.clsph_gnn_adam(self.h, ctypes.c_float(lr), ctypes.c_float(beta1),
+                                          ctypes.c_float(beta2), ctypes.c_float(eps)))
 
 
 class Batch(State):
