@@ -69,6 +69,7 @@ struct BatchParams {
   uint32_t sort_passes;  // LSD passes over the cell-id bits (set by the host)
   uint32_t digit_bits;   // bits per pass: ceil((key_bits - 3) / sort_passes), at most 11
   uint32_t err;
+  uint32_t ticket;       // grid-parameter work-groups done this substep (zero between)
 };
 
 // Work-group descriptor for per-particle kernels: scene and first particle.

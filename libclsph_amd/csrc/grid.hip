@@ -31,10 +31,42 @@ __global__ __launch_bounds__(kMaxBlock) void k_aabb(const float4* __restrict__ p
   block_aabb_partial(p, valid, partial);
 }
 
+// Lays the scenes' half-cell key ranges end to end (each scene gets
+// 8 (G + 1) keys) and sizes the sort digits for the total (one thread).
+__device__ void batch_keys(DevParams* __restrict__ dps, BatchParams* __restrict__ bp) {
+  uint64_t base = 0;
+  uint32_t err = 0;
+  for (uint32_t s = 0; s < bp->n_scenes; ++s) {
+    DevParams* dp = dps + s;
+    uint64_t G = (dp->err & kErrFineKey) ? 0u : dp->p.grid_cell_count;
+    dp->key_base = (uint32_t)base;
+    dp->key_end = (uint32_t)(base + 8ull * G);
+    base += 8ull * (G + 1);
+    if (base > 0xFFFFFFF0ull) {
+      err |= kErrFineKey;
+      base = 0xFFFFFFF0ull;
+    }
+  }
+  if (base > bp->table_cap) err |= kErrTableCap;
+  bp->key_total = (uint32_t)base;
+  uint32_t kb = bitlen((uint32_t)base);
+  bp->key_bits = kb;
+  // the sort orders by cell id (key >> 3) in bp->sort_passes LSD passes
+  uint32_t cb = kb > 3u ? kb - 3u : 1u;
+  uint32_t passes = bp->sort_passes ? bp->sort_passes : 1u;
+  uint32_t db = (cb + passes - 1u) / passes;
+  bp->digit_bits = db < 1u ? 1u : (db > (uint32_t)kMaxDigitBits ? (uint32_t)kMaxDigitBits : db);
+  if (err) atomicOr(&bp->err, err);
+}
+
 // One work-group per scene: reduces the scene's partial boxes, then restates
-// sph_simulation.cpp:201-252 in FP32.
+// sph_simulation.cpp:201-252 in FP32. The last work-group to finish lays out
+// the batch's keys (batch_keys).
+__device__ void scene_grid(DevParams* __restrict__ dp, const float v[6]);
+
 __global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dps,
-                                                      const float* __restrict__ partial) {
+                                                      const float* __restrict__ partial,
+                                                      BatchParams* __restrict__ bp) {
   __shared__ float red[6][16];
   DevParams* dp = dps + blockIdx.x;
   const float inf = __builtin_huge_valf();
@@ -56,6 +88,18 @@ __global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dp
     for (int q = 1; q < nw; ++q) r = (k < 3) ? fminf(r, red[k][q]) : fmaxf(r, red[k][q]);
     v[k] = r;
   }
+  scene_grid(dp, v);
+  // the scenes' grids are in place once every work-group has passed here
+  __threadfence();
+  if (atomicAdd(&bp->ticket, 1u) == gridDim.x - 1u) {
+    __threadfence();
+    bp->ticket = 0;
+    batch_keys(dps, bp);
+  }
+}
+
+// The grid of one scene from its bounding box v = (min xyz, max xyz).
+__device__ void scene_grid(DevParams* __restrict__ dp, const float v[6]) {
   dp->capped = 0;
   if (dp->n == 0) {
     dp->p.grid_cell_count = 0;
@@ -89,35 +133,6 @@ __global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dp
   if (err) atomicOr(&dp->err, err);
 }
 
-// One thread: lays the scenes' half-cell key ranges end to end (each scene
-// gets 8 (G + 1) keys) and sizes the sort digits for the total.
-__global__ void k_batch_keys(DevParams* __restrict__ dps, BatchParams* __restrict__ bp) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint64_t base = 0;
-  uint32_t err = 0;
-  for (uint32_t s = 0; s < bp->n_scenes; ++s) {
-    DevParams* dp = dps + s;
-    uint64_t G = (dp->err & kErrFineKey) ? 0u : dp->p.grid_cell_count;
-    dp->key_base = (uint32_t)base;
-    dp->key_end = (uint32_t)(base + 8ull * G);
-    base += 8ull * (G + 1);
-    if (base > 0xFFFFFFF0ull) {
-      err |= kErrFineKey;
-      base = 0xFFFFFFF0ull;
-    }
-  }
-  if (base > bp->table_cap) err |= kErrTableCap;
-  bp->key_total = (uint32_t)base;
-  uint32_t kb = bitlen((uint32_t)base);
-  bp->key_bits = kb;
-  // the sort orders by cell id (key >> 3) in bp->sort_passes LSD passes
-  uint32_t cb = kb > 3u ? kb - 3u : 1u;
-  uint32_t passes = bp->sort_passes ? bp->sort_passes : 1u;
-  uint32_t db = (cb + passes - 1u) / passes;
-  bp->digit_bits = db < 1u ? 1u : (db > (uint32_t)kMaxDigitBits ? (uint32_t)kMaxDigitBits : db);
-  if (err) atomicOr(&bp->err, err);
-}
-
 // locate_in_grid (grid.cl:43-67): the reference's cell id is
 // Morton((uint)((x - min) / (h * 2))); the half-cell key adds the octant and
 // the scene's key base.
@@ -149,8 +164,7 @@ void launch_aabb(clsph_state* s) {
 
 void launch_grid_params(clsph_state* s) {
   hipLaunchKernelGGL(k_grid_params, dim3(s->n_scenes), dim3(1024), 0, s->ctx->stream, s->dps,
-                     s->aabb_partial);
-  hipLaunchKernelGGL(k_batch_keys, dim3(1), dim3(64), 0, s->ctx->stream, s->dps, s->bp);
+                     s->aabb_partial, s->bp);
 }
 
 void launch_hash(clsph_state* s) {
