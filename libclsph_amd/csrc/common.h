@@ -99,6 +99,22 @@ __host__ __device__ inline uint32_t bitlen(uint32_t v) {
   return b;
 }
 
+// locate_in_grid (grid.cl:43-67): the reference's cell id is
+// Morton((uint)((x - min) / (h * 2))); the half-cell key adds the octant and
+// the scene's key base.
+__device__ inline uint32_t hash_key(float4 p, const DevParams* __restrict__ dp) {
+#pragma clang fp contract(off)
+  float L = dp->p.h * 2;
+  float u[3] = {(p.x - dp->p.min_point.s[0]) / L, (p.y - dp->p.min_point.s[1]) / L,
+                (p.z - dp->p.min_point.s[2]) / L};
+  uint32_t c[3], oct = 0;
+  for (int a = 0; a < 3; ++a) {
+    c[a] = (uint32_t)u[a];
+    oct |= (((uint32_t)(2.f * u[a])) & 1u) << a;
+  }
+  return dp->key_base + ((morton3(c[0], c[1], c[2]) << 3) | oct);
+}
+
 // The learned surrogate's node inputs of a particle: nu = [rho / rho0 - 1,
 // asinh(p / K) / 4].
 __device__ inline float2 node_in(float rho, float p, float rho0, float K) {

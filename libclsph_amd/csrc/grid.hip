@@ -133,9 +133,7 @@ __device__ void scene_grid(DevParams* __restrict__ dp, const float v[6]) {
   if (err) atomicOr(&dp->err, err);
 }
 
-// locate_in_grid (grid.cl:43-67): the reference's cell id is
-// Morton((uint)((x - min) / (h * 2))); the half-cell key adds the octant and
-// the scene's key base.
+// Half-cell keys of every particle (hash_key).
 __global__ __launch_bounds__(kMaxBlock) void k_hash(const float4* __restrict__ pos,
                                                     const BlockDesc* __restrict__ blocks,
                                                     const DevParams* __restrict__ dps,
@@ -143,17 +141,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_hash(const float4* __restrict__ p
   uint32_t scene;
   uint32_t i = particle_of(blocks, dps, scene);
   if (i == 0xFFFFFFFFu) return;
-  const DevParams* dp = dps + scene;
-  float4 p = pos[i];
-  float L = dp->p.h * 2;
-  float u[3] = {(p.x - dp->p.min_point.s[0]) / L, (p.y - dp->p.min_point.s[1]) / L,
-                (p.z - dp->p.min_point.s[2]) / L};
-  uint32_t c[3], oct = 0;
-  for (int a = 0; a < 3; ++a) {
-    c[a] = (uint32_t)u[a];
-    oct |= (((uint32_t)(2.f * u[a])) & 1u) << a;
-  }
-  key[i] = dp->key_base + ((morton3(c[0], c[1], c[2]) << 3) | oct);
+  key[i] = hash_key(pos[i], dps + scene);
 }
 
 void launch_aabb(clsph_state* s) {
