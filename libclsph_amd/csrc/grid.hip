@@ -31,6 +31,22 @@ __global__ __launch_bounds__(kMaxBlock) void k_aabb(const float4* __restrict__ p
   block_aabb_partial(p, valid, partial);
 }
 
+// The batch's key total, key width and sort digit width for `base` keys in
+// all (one thread); `err` holds the error bits raised while laying them out.
+__device__ void batch_layout(BatchParams* __restrict__ bp, uint64_t base, uint32_t err,
+                             uint32_t table_cap, uint32_t sort_passes) {
+  if (base > table_cap) err |= kErrTableCap;
+  bp->key_total = (uint32_t)base;
+  uint32_t kb = bitlen((uint32_t)base);
+  bp->key_bits = kb;
+  // the sort orders by cell id (key >> 3) in sort_passes LSD passes
+  uint32_t cb = kb > 3u ? kb - 3u : 1u;
+  uint32_t passes = sort_passes ? sort_passes : 1u;
+  uint32_t db = (cb + passes - 1u) / passes;
+  bp->digit_bits = db < 1u ? 1u : (db > (uint32_t)kMaxDigitBits ? (uint32_t)kMaxDigitBits : db);
+  if (err) atomicOr(&bp->err, err);
+}
+
 // Lays the scenes' half-cell key ranges end to end (each scene gets
 // 8 (G + 1) keys) and sizes the sort digits for the total (one thread).
 __device__ void batch_keys(DevParams* __restrict__ dps, BatchParams* __restrict__ bp) {
@@ -47,33 +63,42 @@ __device__ void batch_keys(DevParams* __restrict__ dps, BatchParams* __restrict_
       base = 0xFFFFFFF0ull;
     }
   }
-  if (base > bp->table_cap) err |= kErrTableCap;
-  bp->key_total = (uint32_t)base;
-  uint32_t kb = bitlen((uint32_t)base);
-  bp->key_bits = kb;
-  // the sort orders by cell id (key >> 3) in bp->sort_passes LSD passes
-  uint32_t cb = kb > 3u ? kb - 3u : 1u;
-  uint32_t passes = bp->sort_passes ? bp->sort_passes : 1u;
-  uint32_t db = (cb + passes - 1u) / passes;
-  bp->digit_bits = db < 1u ? 1u : (db > (uint32_t)kMaxDigitBits ? (uint32_t)kMaxDigitBits : db);
-  if (err) atomicOr(&bp->err, err);
+  batch_layout(bp, base, err, bp->table_cap, bp->sort_passes);
 }
 
 // One work-group per scene: reduces the scene's partial boxes, then restates
 // sph_simulation.cpp:201-252 in FP32. The last work-group to finish lays out
-// the batch's keys (batch_keys).
-__device__ void scene_grid(DevParams* __restrict__ dp, const float v[6]);
+// the batch's keys (batch_keys); a single scene does so from the values in
+// its registers. The parameters thread 0 needs are loaded before the
+// reduction, so that their latency overlaps it.
+__device__ uint32_t scene_grid(DevParams* __restrict__ dp, const float v[6], float h, uint32_t n,
+                               uint32_t& G);
 
 __global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dps,
                                                       const float* __restrict__ partial,
                                                       BatchParams* __restrict__ bp) {
   __shared__ float red[6][16];
   DevParams* dp = dps + blockIdx.x;
+  const float h = dp->p.h;
+  const uint32_t np = dp->n, err0 = dp->err;
+  const uint32_t table_cap = bp->table_cap, sort_passes = bp->sort_passes;
   const float inf = __builtin_huge_valf();
   float v[6] = {inf, inf, inf, -inf, -inf, -inf};
-  for (uint32_t b = dp->block_begin + threadIdx.x; b < dp->block_end; b += blockDim.x) {
-    for (int k = 0; k < 3; ++k) v[k] = fminf(v[k], partial[b * 6 + k]);
-    for (int k = 3; k < 6; ++k) v[k] = fmaxf(v[k], partial[b * 6 + k]);
+  // four partial boxes per thread in flight at a time
+  const uint32_t b0 = dp->block_begin, b1 = dp->block_end;
+  for (uint32_t b = b0 + threadIdx.x; b < b1; b += 4u * blockDim.x) {
+    float q[4][6];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t bu = b + (uint32_t)u * blockDim.x;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) q[u][k] = bu < b1 ? partial[bu * 6 + k] : (k < 3 ? inf : -inf);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      for (int k = 0; k < 3; ++k) v[k] = fminf(v[k], q[u][k]);
+      for (int k = 3; k < 6; ++k) v[k] = fmaxf(v[k], q[u][k]);
+    }
   }
   for (int k = 0; k < 3; ++k) v[k] = wave_min(v[k]);
   for (int k = 3; k < 6; ++k) v[k] = wave_max(v[k]);
@@ -88,7 +113,22 @@ __global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dp
     for (int q = 1; q < nw; ++q) r = (k < 3) ? fminf(r, red[k][q]) : fmaxf(r, red[k][q]);
     v[k] = r;
   }
-  scene_grid(dp, v);
+  uint32_t G;
+  const uint32_t err = scene_grid(dp, v, h, np, G);
+  if (gridDim.x == 1u) {
+    // one scene: batch_keys with the scene's values in registers
+    const uint64_t Gk = ((err0 | err) & kErrFineKey) ? 0u : G;
+    dp->key_base = 0;
+    dp->key_end = (uint32_t)(8ull * Gk);
+    uint64_t base = 8ull * (Gk + 1);
+    uint32_t berr = 0;
+    if (base > 0xFFFFFFF0ull) {
+      berr = kErrFineKey;
+      base = 0xFFFFFFF0ull;
+    }
+    batch_layout(bp, base, berr, table_cap, sort_passes);
+    return;
+  }
   // the scenes' grids are in place once every work-group has passed here
   __threadfence();
   if (atomicAdd(&bp->ticket, 1u) == gridDim.x - 1u) {
@@ -98,17 +138,21 @@ __global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dp
   }
 }
 
-// The grid of one scene from its bounding box v = (min xyz, max xyz).
-__device__ void scene_grid(DevParams* __restrict__ dp, const float v[6]) {
+// The grid of one scene from its bounding box v = (min xyz, max xyz), its
+// smoothing length h and particle count n; returns the error bits it raised
+// and the cell count in G.
+__device__ uint32_t scene_grid(DevParams* __restrict__ dp, const float v[6], float h, uint32_t n,
+                               uint32_t& G) {
   dp->capped = 0;
-  if (dp->n == 0) {
+  G = 0;
+  if (n == 0) {
     dp->p.grid_cell_count = 0;
-    return;
+    return 0u;
   }
   // Initial values are numeric_limits<cl_int>::max()/min() as float; a
   // position replaces them only when strictly smaller / larger.
   const float init_min = 2147483648.0f, init_max = -2147483648.0f;
-  float L = dp->p.h * 2;
+  float L = h * 2;
   float mn[3], mx[3];
   uint32_t gs[3];
   uint32_t err = 0;
@@ -127,10 +171,11 @@ __device__ void scene_grid(DevParams* __restrict__ dp, const float v[6]) {
   dp->p.grid_size_x = (int32_t)gs[0];
   dp->p.grid_size_y = (int32_t)gs[1];
   dp->p.grid_size_z = (int32_t)gs[2];
-  uint32_t G = morton3(gs[0], gs[1], gs[2]);
+  G = morton3(gs[0], gs[1], gs[2]);
   dp->p.grid_cell_count = G;
   if (G >= (1u << 29)) err |= kErrFineKey;
   if (err) atomicOr(&dp->err, err);
+  return err;
 }
 
 // Half-cell keys of every particle (hash_key).
