@@ -80,15 +80,22 @@ def roofline(dom, achieved, ms, workload, n):
     """The dominant kernel against the HBM peak (algorithmic bytes over its
     measured duration) and, from the committed counters of the same sources,
     its measured HBM traffic and its VALU issue fraction (VALU instructions x
-    4 cycles over the SIMDs' cycles in its duration): the bound it meets."""
+    4 cycles over the SIMDs' cycles in its duration): the bound it meets.
+    The counter fractions are taken over the kernel's duration in the
+    counter run itself (its sample of substeps), not over this run's."""
     c = measured_counters(dom, workload, n)
     out = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": c.get("traffic")}
+    cms = c["us"] * 1e-3 if c.get("us") else ms
     if c.get("traffic"):
-        out["traffic_frac"] = c["traffic"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+        out["traffic_frac"] = c["traffic"] / (cms * 1e-3) / 1e9 / HBM_PEAK_GBS
     if c.get("valu"):
-        out["valu_issue_frac"] = c["valu"] * 4.0 / (SIMDS * CLOCK_HZ * ms * 1e-3)
+        out["valu_issue_frac"] = c["valu"] * 4.0 / (SIMDS * CLOCK_HZ * cms * 1e-3)
         out["limiter"] = "valu" if out["valu_issue_frac"] > out.get("traffic_frac", 0.0) else "hbm"
+    if c:
+        from libclsph_amd import counters
+        out["counter_sample"] = counters.load().get("sample")
+        out["counter_kernel_ms"] = cms
     return out
 
 

@@ -7,6 +7,7 @@ launched stage by stage with identity tracking on, and writes
 profiles/counters.json:
 
   stamp       source_stamp() of the build the counters were taken on
+  sample      the substeps the counters were taken on
   kernels     per kernel (name without template arguments and parameters):
                 traffic     HBM bytes per launch: FETCH_SIZE and WRITE_SIZE
                             (KiB per dispatch) scaled by what they read for
@@ -14,6 +15,9 @@ profiles/counters.json:
                             particle
                 valu        VALU instructions per launch (all waves)
                 waves       waves per launch
+                us          mean duration of a launch in the SQ run (its
+                            dispatch timestamps), the time the fractions of
+                            these counters are taken over
 
 bench.py uses an entry only while the stamp matches the sources it runs.
 
@@ -65,6 +69,19 @@ def per_dispatch(path):
     return {k: {c: v / len(disp[k][c]) for c, v in cs.items()} for k, cs in sums.items()}
 
 
+def durations(path):
+    """{kernel: mean dispatch duration in microseconds} of a counter CSV."""
+    span = {}
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            span[row["Dispatch_Id"]] = (kernel_key(row["Kernel_Name"]),
+                                        int(row["End_Timestamp"]) - int(row["Start_Timestamp"]))
+    acc = collections.defaultdict(list)
+    for k, d in span.values():
+        acc[k].append(d)
+    return {k: sum(v) / len(v) / 1e3 for k, v in acc.items()}
+
+
 def load(stamp=None):
     """The committed counters, or None when absent or taken on other sources."""
     if not os.path.exists(COUNTERS):
@@ -82,9 +99,11 @@ def main():
     ap.add_argument("write")
     ap.add_argument("sq")
     ap.add_argument("--particles", type=int, default=1048576)
+    ap.add_argument("--sample", default="substeps 6-15 of the dam break, launched stage by stage")
     ap.add_argument("--out", default=COUNTERS)
     a = ap.parse_args()
     fetch, write, sq = per_dispatch(a.fetch), per_dispatch(a.write), per_dispatch(a.sq)
+    us = durations(a.sq)
     copy_bytes = 4.0 * a.particles
     fs = copy_bytes / (fetch["k_copy_u32"]["FETCH_SIZE"] * 1024.0)
     ws = copy_bytes / (write["k_copy_u32"]["WRITE_SIZE"] * 1024.0)
@@ -96,8 +115,9 @@ def main():
         if k in sq:
             e["valu"] = sq[k].get("SQ_INSTS_VALU")
             e["waves"] = sq[k].get("SQ_WAVES")
+            e["us"] = us.get(k)
         kernels[k] = e
-    out = {"stamp": source_stamp(), "workload": f"dam_break {a.particles}",
+    out = {"stamp": source_stamp(), "workload": f"dam_break {a.particles}", "sample": a.sample,
            "fetch_scale": fs, "write_scale": ws, "kernels": kernels}
     with open(a.out, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
