@@ -20,7 +20,12 @@ __device__ inline uint16_t to_bf16(float x) {
 __device__ inline float from_bf16(uint16_t b) {
   return __builtin_bit_cast(float, (uint32_t)b << 16);
 }
-__device__ inline float sigmoid(float x) { return 1.f / (1.f + __expf(-x)); }
+// Logistic function from the hardware exp2 and reciprocal (1 ulp each): the
+// activations it feeds are rounded to bf16, so IEEE division would only add
+// instructions.
+__device__ inline float sigmoid(float x) {
+  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(x * -1.44269504f));
+}
 __device__ inline float silu(float x) { return x * sigmoid(x); }
 __device__ inline float dsilu(float x) {
   float s = sigmoid(x);
@@ -187,7 +192,7 @@ __device__ inline u32x4 pack8(const float v[8]) {
 }
 
 template <bool ROUND2>
-__global__ __launch_bounds__(256) void k_edge_fused(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_edge_fused(
     const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float* __restrict__ rho, const float* __restrict__ prs,
     const uint4* __restrict__ nbr, const uint32_t* __restrict__ count, uint32_t n, float h,
@@ -300,6 +305,7 @@ __global__ __launch_bounds__(256) void k_edge_fused(
           float v = silu(y[r] + bs[kWidth + 16 * ob + 4 * g + r]);
           acc[ob][r] += valid ? v : 0.f;
         }
+        __builtin_amdgcn_sched_barrier(0);  // one output block's reads at a time
       }
     }
     // sum over the 16 edge lanes of each group
@@ -325,6 +331,263 @@ __global__ __launch_bounds__(256) void k_edge_fused(
       degc[i] = (float)deg / kAggC;
       nu[2 * i] = ni.x;
       nu[2 * i + 1] = ni.y;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused node MLPs for inference. A wave handles tiles of 16 particles; each
+// layer runs transposed on 16x16x32 bf16 matrix cores (outputs as rows,
+// particles as columns), chaining in registers as the edge kernel does. B
+// operands come either from memory in the natural K order (slot t of k-step
+// ks, lane group g: feature 32 ks + 8 g + t) or from the previous layer's
+// registers in the permuted order (32 ks + 16 (t / 4) + 4 g + t % 4); each
+// layer's weights are staged once per work-group in LDS in the matching
+// order. Rounding points are those of the GEMM path (bf16 layer inputs,
+// float32 accumulation).
+constexpr int kNodeTile = 16;
+
+// Stages W rows for k-steps [ks0, ks0 + nks) of a layer with `ks_total`
+// k-steps and `nob` 16-wide output blocks: dst[(ob * ks_total + ks) * 64 +
+// lane] holds W[k][16 ob + lane % 16] for the lane's 8 K slots; rows at or
+// beyond `kvalid` are zero. `krow0` is the W row of the first staged k-step.
+__device__ __attribute__((always_inline)) inline void stage_layer(u32x4* dst, int ks_total, int ks0, int nks, int nob,
+                                   const uint16_t* __restrict__ Wm, int ld, int krow0, int kvalid,
+                                   bool perm) {
+  const int total = nob * nks * 64;
+  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    int l = idx & 63, ks = (idx >> 6) % nks, ob = (idx >> 6) / nks, g = l >> 4;
+    int col = 16 * ob + (l & 15);
+    uint32_t w[4];
+    for (int t2 = 0; t2 < 4; ++t2) {
+      uint32_t half[2];
+      for (int h = 0; h < 2; ++h) {
+        int t = 2 * t2 + h;
+        int k = krow0 + 32 * ks + (perm ? 16 * (t >> 2) + 4 * g + (t & 3) : 8 * g + t);
+        half[h] = k < kvalid ? Wm[(size_t)k * ld + col] : 0u;
+      }
+      w[t2] = half[0] | (half[1] << 16);
+    }
+    dst[(ob * ks_total + ks0 + ks) * 64 + l] = u32x4{w[0], w[1], w[2], w[3]};
+  }
+}
+
+template <int NOB, int NKS>
+__device__ __attribute__((always_inline)) inline void mfma_layer(const u32x4* __restrict__ w, const bf16x8* bop, f32x4* y) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int ob = 0; ob < NOB; ++ob) {
+    f32x4 a = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks)
+      a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, w[(ob * NKS + ks) * 64 + lane]),
+                                                   bop[ks], a, 0, 0, 0);
+    y[ob] = a;
+    // keep each block's weight reads next to its products: hoisting all of
+    // a layer's LDS reads would need 4 NOB NKS registers more
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Permuted-order B operands (4 k-steps, 128 features) from a 128-wide
+// layer's registers, optionally through SiLU.
+__device__ __attribute__((always_inline)) inline void to_operands(const f32x4* y, bool act, bf16x8* bop) {
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) {
+    float v8[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      float v = y[2 * ks + (t >> 2)][t & 3];
+      v8[t] = act ? silu(v) : v;
+    }
+    bop[ks] = __builtin_bit_cast(bf16x8, pack8(v8));
+  }
+}
+
+// Natural-order B operands: features 32 ks + 8 g .. + 7 of a bf16 row.
+__device__ __attribute__((always_inline)) inline void load_operands(const uint16_t* __restrict__ row, bool valid, int nks,
+                                     bf16x8* bop) {
+  const int g = (threadIdx.x & 63) >> 4;
+  for (int ks = 0; ks < nks; ++ks) {
+    u32x4 v = {0u, 0u, 0u, 0u};
+    if (valid) v = *(const u32x4*)(row + 32 * ks + 8 * g);
+    bop[ks] = __builtin_bit_cast(bf16x8, v);
+  }
+}
+
+// Adds bias[16 ob + 4 g + r] (+ scale * bias2) to a 128-wide layer.
+__device__ __attribute__((always_inline)) inline void add_bias(f32x4* y, int nob, const float* __restrict__ b,
+                                const float* __restrict__ b2, float scale) {
+  const int g = (threadIdx.x & 63) >> 4;
+  for (int ob = 0; ob < nob; ++ob)
+    for (int r = 0; r < 4; ++r) {
+      int f = 16 * ob + 4 * g + r;
+      y[ob][r] += b[f] + (b2 ? scale * b2[f] : 0.f);
+    }
+}
+
+// Writes features 16 ob + 4 g .. + 3 of a particle's bf16 row.
+__device__ __attribute__((always_inline)) inline void store_row_bf16(uint16_t* __restrict__ row, const f32x4* y) {
+  const int g = (threadIdx.x & 63) >> 4;
+  for (int ob = 0; ob < 8; ++ob) {
+    uint32_t lo = (uint32_t)to_bf16(y[ob][0]) | ((uint32_t)to_bf16(y[ob][1]) << 16);
+    uint32_t hi = (uint32_t)to_bf16(y[ob][2]) | ((uint32_t)to_bf16(y[ob][3]) << 16);
+    *(uint2*)(row + 16 * ob + 4 * g) = make_uint2(lo, hi);
+  }
+}
+
+// Round-1 node part: M1 = S0 W3 + (deg / C) b3; H = MLP([M1, nu, 0]); Hb.
+__global__ __launch_bounds__(512) void k_node1(const uint16_t* __restrict__ S0,
+                                               const float* __restrict__ degc,
+                                               const float* __restrict__ nu, uint32_t n,
+                                               const uint16_t* __restrict__ wb,
+                                               const float* __restrict__ params,
+                                               uint16_t* __restrict__ Hb) {
+  extern __shared__ u32x4 lds[];
+  u32x4* w3 = lds;                 // 8 x 4
+  u32x4* wa = w3 + 8 * 4 * 64;     // 8 x 5
+  u32x4* wb2 = wa + 8 * 5 * 64;    // 8 x 4
+  u32x4* wc = wb2 + 8 * 4 * 64;    // 8 x 4
+  float* bias = (float*)(wc + 8 * 4 * 64);  // b3, ba, bb, bc
+  stage_layer(w3, 4, 0, 4, 8, wb + param_offset(E1_W3), kWidth, 0, kWidth, false);
+  stage_layer(wa, 5, 0, 4, 8, wb + param_offset(N1_W1), kWidth, 0, kIn1, true);
+  stage_layer(wa, 5, 4, 1, 8, wb + param_offset(N1_W1), kWidth, kWidth, kWidth + 2, false);
+  stage_layer(wb2, 4, 0, 4, 8, wb + param_offset(N1_W2), kWidth, 0, kWidth, true);
+  stage_layer(wc, 4, 0, 4, 8, wb + param_offset(N1_W3), kWidth, 0, kWidth, true);
+  for (int k = threadIdx.x; k < kWidth; k += blockDim.x) {
+    bias[k] = params[param_offset(E1_B3) + k];
+    bias[kWidth + k] = params[param_offset(N1_B1) + k];
+    bias[2 * kWidth + k] = params[param_offset(N1_B2) + k];
+    bias[3 * kWidth + k] = params[param_offset(N1_B3) + k];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  const uint32_t tiles = (n + kNodeTile - 1) / kNodeTile;
+  for (uint32_t tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < tiles;
+       tile += waves) {
+    const uint32_t i = tile * kNodeTile + c;
+    const bool valid = i < n;
+    bf16x8 bop[5];
+    f32x4 y[8];
+    load_operands(S0 + (size_t)(valid ? i : 0) * kWidth, valid, 4, bop);
+    mfma_layer<8, 4>(w3, bop, y);
+    {
+      const float dc = valid ? degc[i] : 0.f;  // M1 = S0 W3 + (deg / C) b3
+      for (int ob = 0; ob < 8; ++ob)
+        for (int r = 0; r < 4; ++r) y[ob][r] += dc * bias[16 * ob + 4 * g + r];
+    }
+    to_operands(y, false, bop);
+    {
+      float v8[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      if (g == 0 && valid) {
+        v8[0] = nu[2 * i];
+        v8[1] = nu[2 * i + 1];
+      }
+      bop[4] = __builtin_bit_cast(bf16x8, pack8(v8));
+    }
+    mfma_layer<8, 5>(wa, bop, y);
+    add_bias(y, 8, bias + kWidth, nullptr, 0.f);
+    to_operands(y, true, bop);
+    mfma_layer<8, 4>(wb2, bop, y);
+    add_bias(y, 8, bias + 2 * kWidth, nullptr, 0.f);
+    to_operands(y, true, bop);
+    mfma_layer<8, 4>(wc, bop, y);
+    add_bias(y, 8, bias + 3 * kWidth, nullptr, 0.f);
+    if (valid) store_row_bf16(Hb + (size_t)i * kWidth, y);
+  }
+}
+
+// Round-2 node terms: PA = bf16(Hb Wa), PB = bf16(Hb Wb).
+__global__ __launch_bounds__(512) void k_node_terms(const uint16_t* __restrict__ Hb, uint32_t n,
+                                                    const uint16_t* __restrict__ wb,
+                                                    uint16_t* __restrict__ PA,
+                                                    uint16_t* __restrict__ PB) {
+  extern __shared__ u32x4 lds[];
+  u32x4* wa = lds;
+  u32x4* wbb = wa + 8 * 4 * 64;
+  stage_layer(wa, 4, 0, 4, 8, wb + param_offset(E2_WA), kWidth, 0, kWidth, false);
+  stage_layer(wbb, 4, 0, 4, 8, wb + param_offset(E2_WB), kWidth, 0, kWidth, false);
+  __syncthreads();
+  const int c = threadIdx.x & 15;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  const uint32_t tiles = (n + kNodeTile - 1) / kNodeTile;
+  for (uint32_t tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < tiles;
+       tile += waves) {
+    const uint32_t i = tile * kNodeTile + c;
+    const bool valid = i < n;
+    bf16x8 bop[4];
+    f32x4 y[8];
+    load_operands(Hb + (size_t)(valid ? i : 0) * kWidth, valid, 4, bop);
+    mfma_layer<8, 4>(wa, bop, y);
+    if (valid) store_row_bf16(PA + (size_t)i * kWidth, y);
+    mfma_layer<8, 4>(wbb, bop, y);
+    if (valid) store_row_bf16(PB + (size_t)i * kWidth, y);
+  }
+}
+
+// Round-2 node part: M2 = S1 W3' + (deg / C) b3'; out = MLP([H, M2]);
+// acc = g + A out[0:3]. Writes out (32 floats per particle) as well.
+__global__ __launch_bounds__(512) void k_node2(const uint16_t* __restrict__ S1,
+                                               const uint16_t* __restrict__ Hb,
+                                               const float* __restrict__ degc, uint32_t n,
+                                               const uint16_t* __restrict__ wb,
+                                               const float* __restrict__ params,
+                                               float* __restrict__ out, float4* __restrict__ acc,
+                                               float gx, float gy, float gz) {
+  extern __shared__ u32x4 lds[];
+  u32x4* w3 = lds;                 // 8 x 4
+  u32x4* wa = w3 + 8 * 4 * 64;     // 8 x 8
+  u32x4* wb2 = wa + 8 * 8 * 64;    // 8 x 4
+  u32x4* wc = wb2 + 8 * 4 * 64;    // 2 x 4
+  float* bias = (float*)(wc + 2 * 4 * 64);  // b3, ba, bb (128 each), bc (32)
+  stage_layer(w3, 4, 0, 4, 8, wb + param_offset(E2_W3), kWidth, 0, kWidth, false);
+  stage_layer(wa, 8, 0, 4, 8, wb + param_offset(N2_W1), kWidth, 0, kWidth, false);
+  stage_layer(wa, 8, 4, 4, 8, wb + param_offset(N2_W1), kWidth, kWidth, kIn2, true);
+  stage_layer(wb2, 4, 0, 4, 8, wb + param_offset(N2_W2), kWidth, 0, kWidth, true);
+  stage_layer(wc, 4, 0, 4, 2, wb + param_offset(N2_W3), kOut, 0, kWidth, true);
+  for (int k = threadIdx.x; k < kWidth; k += blockDim.x) {
+    bias[k] = params[param_offset(E2_B3) + k];
+    bias[kWidth + k] = params[param_offset(N2_B1) + k];
+    bias[2 * kWidth + k] = params[param_offset(N2_B2) + k];
+    if (k < kOut) bias[3 * kWidth + k] = params[param_offset(N2_B3) + k];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  const uint32_t tiles = (n + kNodeTile - 1) / kNodeTile;
+  for (uint32_t tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < tiles;
+       tile += waves) {
+    const uint32_t i = tile * kNodeTile + c;
+    const bool valid = i < n;
+    bf16x8 bop[8];
+    f32x4 y[8];
+    load_operands(S1 + (size_t)(valid ? i : 0) * kWidth, valid, 4, bop);
+    mfma_layer<8, 4>(w3, bop, y);
+    {
+      const float dc = valid ? degc[i] : 0.f;
+      for (int ob = 0; ob < 8; ++ob)
+        for (int r = 0; r < 4; ++r) y[ob][r] += dc * bias[16 * ob + 4 * g + r];
+    }
+    load_operands(Hb + (size_t)(valid ? i : 0) * kWidth, valid, 4, bop);
+    to_operands(y, false, bop + 4);
+    mfma_layer<8, 8>(wa, bop, y);
+    add_bias(y, 8, bias + kWidth, nullptr, 0.f);
+    to_operands(y, true, bop);
+    mfma_layer<8, 4>(wb2, bop, y);
+    add_bias(y, 8, bias + 2 * kWidth, nullptr, 0.f);
+    to_operands(y, true, bop);
+    f32x4 o[2];
+    mfma_layer<2, 4>(wc, bop, o);
+    for (int ob = 0; ob < 2; ++ob)
+      for (int r = 0; r < 4; ++r) o[ob][r] += bias[3 * kWidth + 16 * ob + 4 * g + r];
+    if (valid) {
+      float* orow = out + (size_t)i * kOut;
+      for (int ob = 0; ob < 2; ++ob)
+        *(float4*)(orow + 16 * ob + 4 * g) = make_float4(o[ob][0], o[ob][1], o[ob][2], o[ob][3]);
+      if (g == 0)
+        acc[i] = make_float4(gx + kAccScale * o[0][0], gy + kAccScale * o[0][1],
+                             gz + kAccScale * o[0][2], 0.f);
     }
   }
 }
@@ -678,9 +941,9 @@ void forward_fused(clsph_gnn* m, clsph_state* st, float4* acc, float gx, float g
   uint32_t n = st->n;
   m->n = n;
   const DevParams& d = st->host_dps[0];
-  // persistent grid: the work-groups that fit at once (two per compute unit
-  // at the kernels' register use) stage the weights once each
-  uint32_t blocks = (uint32_t)st->ctx->cus * 2;
+  // persistent grid: the work-groups that fit at once (three per compute
+  // unit at the kernels' register and LDS use) stage the weights once each
+  uint32_t blocks = (uint32_t)st->ctx->cus * 3;
   if (blocks * 4 > n) blocks = (n + 3) / 4;
   if (blocks == 0) blocks = 1;
   if (ev) (void)hipEventRecord(ev[0], s);
@@ -689,14 +952,37 @@ void forward_fused(clsph_gnn* m, clsph_state* st, float4* acc, float gx, float g
                      d.p.K, W(m, E1_W1), P(m, E1_B1), W(m, E1_W2), P(m, E1_B2),
                      (const uint16_t*)nullptr, (const uint16_t*)nullptr, m->S[0], m->degc, m->nu);
   if (ev) (void)hipEventRecord(ev[1], s);
-  node_round1(m, s);
+  // node part of round 1 (fused)
+  constexpr size_t kLdsNode1 = (size_t)(8 * 4 + 8 * 5 + 8 * 4 + 8 * 4) * 64 * 16 + 4 * kWidth * 4;
+  constexpr size_t kLdsTerms = (size_t)(8 * 4 + 8 * 4) * 64 * 16;
+  constexpr size_t kLdsNode2 =
+      (size_t)(8 * 4 + 8 * 8 + 8 * 4 + 2 * 4) * 64 * 16 + (3 * kWidth + kOut) * 4;
+  static bool lds_set = false;
+  if (!lds_set) {
+    (void)hipFuncSetAttribute((const void*)k_node1, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kLdsNode1);
+    (void)hipFuncSetAttribute((const void*)k_node_terms,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsTerms);
+    (void)hipFuncSetAttribute((const void*)k_node2, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kLdsNode2);
+    lds_set = true;
+  }
+  uint32_t tiles = (n + kNodeTile - 1) / kNodeTile;
+  uint32_t nblocks = (uint32_t)st->ctx->cus;
+  if (nblocks * 8 > tiles) nblocks = (tiles + 7) / 8;
+  if (nblocks == 0) nblocks = 1;
+  hipLaunchKernelGGL(k_node1, dim3(nblocks), dim3(512), kLdsNode1, s, m->S[0], m->degc, m->nu, n,
+                     m->wb, m->params, m->Hb);
+  hipLaunchKernelGGL(k_node_terms, dim3(nblocks), dim3(512), kLdsTerms, s, m->Hb, n, m->wb, m->PA,
+                     m->PB);
   if (ev) (void)hipEventRecord(ev[2], s);
   hipLaunchKernelGGL((k_edge_fused<true>), dim3(blocks), dim3(256), 0, s, st->pos_s, st->vel_s,
                      st->rho_s, st->prs_s, st->nbr, st->nbr_count, n, d.p.h, d.p.fluid_density,
                      d.p.K, W(m, E2_W1), P(m, E2_B1), W(m, E2_W2), P(m, E2_B2), m->PA, m->PB,
                      m->S[1], m->degc, m->nu);
   if (ev) (void)hipEventRecord(ev[3], s);
-  node_round2(m, acc, gx, gy, gz, s);
+  hipLaunchKernelGGL(k_node2, dim3(nblocks), dim3(512), kLdsNode2, s, m->S[1], m->Hb, m->degc, n,
+                     m->wb, m->params, m->out, acc, gx, gy, gz);
   if (ev) (void)hipEventRecord(ev[4], s);
 }
 
