@@ -114,3 +114,49 @@ def test_pour_series_against_oracle(ctx):
     rel_ke = np.abs(np.array(ke) / g["kinetic_energy"] - 1)
     print(f"pour: max rel mean-density diff {rel_rho.max():.2e}, KE diff {rel_ke.max():.2e}")
     assert rel_rho.max() <= 5e-3 and rel_ke.max() <= 5e-3
+
+
+def contact_slab(n_side=32, layers=16, seed=7):
+    """A slab at the 4,194,304-particle pour's spacing (s = cbrt(0.128 / 4M),
+    h = 2 s) and step (dt 5e-5): n_side x n_side columns over the middle of
+    the terrain, `layers` deep, each column's bottom particle 1-40 um above
+    the surface, moving down at 1 m/s with a sideways drift, so the
+    bottom layer meets the terrain in the first substep."""
+    rng = np.random.default_rng(seed)
+    normals, verts, idx = scenes.terrain()
+    s = (0.128 / 4194304) ** (1.0 / 3.0)
+    h = 2.0 * s
+    n = n_side * n_side * layers
+    g = 0.5 + (np.arange(n_side) - n_side / 2 + 0.5) * s
+    X, Z = np.meshgrid(g, g, indexing="ij")
+    x = np.tile(X.ravel(), layers)
+    z = np.tile(Z.ravel(), layers)
+    k = np.repeat(np.arange(layers), n_side * n_side)
+    y = terrain_height(verts, 128, x, z) + rng.uniform(1e-6, 4e-5, n) + k * s
+    parts = layout.particles(n)
+    parts["position"][:, 0] = x
+    parts["position"][:, 1] = y
+    parts["position"][:, 2] = z
+    v = np.stack([rng.uniform(-0.1, 0.1, n), np.full(n, -1.0), rng.uniform(-0.1, 0.1, n)], axis=1)
+    parts["velocity"][:, :3] = v
+    parts["intermediate_velocity"][:, :3] = v
+    p = scenes.fluid_params(n, np.float32(h), np.float32(1000.0 * s ** 3), dt=5e-5)
+    p["restitution"] = np.float32(0.3)
+    return scenes.Scene(parts, p, native.kernel_terms(float(np.float32(h))), normals, verts, idx)
+
+
+def test_terrain_contact_at_the_pour_step_against_oracle(ctx):
+    """One substep at the pour's spacing and step with the bottom layer meeting
+    the 32,768-triangle terrain: grid, cell ids, stable order and cell table
+    bit-exact, density, forces and the advected state (collisions included)
+    within the stage tolerances, against the oracle testing every triangle."""
+    from test_gpu_stages import check_against_oracle, run_gpu_stages
+    sc = contact_slab()
+    print(f"case terrain contact slab {sc.n} particles, dt 5e-5", flush=True)
+    ref = oracle.stages(sc.particles, sc.params, sc.terms, sc.normals, sc.vertices, sc.indices)
+    assert ref.grid_rc == 0
+    hits = int(np.sum(ref.advected["intermediate_velocity"][:, 1] > -0.5))
+    assert hits >= 900, f"the bottom layer (1,024 particles) should meet the terrain ({hits} hits)"
+    g = run_gpu_stages(ctx, sc)
+    check_against_oracle(sc.particles, sc.params, g, ref)
+    print(f"{hits} terrain hits")
