@@ -130,15 +130,19 @@ __device__ inline void for_each_run(const uint32_t hc[3], const float fr[3], uin
 // Density sum and neighbour list of one particle i. The list holds the
 // neighbours (r < h, self excluded) in visiting order, four to a uint4, at
 // nbr[q * n + i] (q < kMaxNeighbours / 4); pending entries wait in a shift
-// register whose newest entry is p3.
+// register whose newest entry is p3, and `row` points at the list row the
+// next four go to (nbr + i, then one row of n entries further per store).
 struct DensityWalk {
   float acc = 0.f;
   uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0, cnt = 0;
+  uint4* row;
+  uint32_t n;
+
+  __device__ DensityWalk(uint4* __restrict__ nbr, uint32_t i, uint32_t n_) : row(nbr + i), n(n_) {}
 
   // Candidate j at squared distance d2; in_run is false for the padding
   // entries of a group of four.
-  __device__ inline void visit(bool in_run, float d2, float h2, uint32_t j, uint32_t i,
-                               uint4* __restrict__ nbr, uint32_t n) {
+  __device__ inline void visit(bool in_run, float d2, float h2, uint32_t j, uint32_t i) {
     static_assert((kMaxNeighbours & (kMaxNeighbours - 1u)) == 0u && kMaxNeighbours >= 4u,
                   "the list-room test below needs a power of two");
     if (in_run && d2 < h2) {
@@ -151,8 +155,10 @@ struct DensityWalk {
       p1 = app ? p2 : p1;
       p2 = app ? p3 : p2;
       p3 = app ? j : p3;
-      if (app && (cnt & (3u - kMaxNeighbours)) == 3u)
-        nbr[(size_t)(cnt >> 2) * n + i] = make_uint4(p0, p1, p2, p3);
+      if (app && (cnt & (3u - kMaxNeighbours)) == 3u) {
+        *row = make_uint4(p0, p1, p2, p3);
+        row += n;
+      }
       cnt += app ? 1u : 0u;
     }
   }
@@ -167,7 +173,7 @@ __device__ inline void density_walk_global(DensityWalk& w, uint32_t i, float4 xi
                                            const DevParams* __restrict__ dp, uint32_t F,
                                            const float4* __restrict__ pos,
                                            const uint32_t* __restrict__ table, float h2,
-                                           uint4* __restrict__ nbr, uint32_t n) {
+                                           uint32_t n) {
   for_each_run(hc, fr, dp->key_base, table, F, n, [&](uint32_t s, uint32_t e) {
     for (uint32_t j0 = s; j0 < e; j0 += 4) {
       const float4* pj = pos + j0;
@@ -178,7 +184,7 @@ __device__ inline void density_walk_global(DensityWalk& w, uint32_t i, float4 xi
       for (int u = 0; u < 4; ++u) {
         const uint32_t j = j0 + u;
         float dx = xi.x - c[u].x, dy = xi.y - c[u].y, dz = xi.z - c[u].z;
-        w.visit(j < e, dx * dx + dy * dy + dz * dz, h2, j, i, nbr, n);
+        w.visit(j < e, dx * dx + dy * dy + dz * dz, h2, j, i);
       }
     }
   });
@@ -192,8 +198,7 @@ __device__ inline void density_walk_global(DensityWalk& w, uint32_t i, float4 xi
 // so every sum and list, is that of density_walk_global.
 __device__ inline void density_walk_halo(DensityWalk& w, uint32_t i, float4 xi,
                                          const uint32_t hc[3], const float fr[3],
-                                         const float4* cand, const uint32_t* off, float h2,
-                                         uint4* __restrict__ nbr, uint32_t n) {
+                                         const float4* cand, const uint32_t* off, float h2) {
   const uint32_t lx = 1u + (hc[0] & 1u), ly = 1u + (hc[1] & 1u), lz = 1u + (hc[2] & 1u);
   const float lo_x = fr[0] * fr[0], hi_x = (1.f - fr[0]) * (1.f - fr[0]);
   for (int dz = -1; dz <= 1; ++dz) {
@@ -215,7 +220,7 @@ __device__ inline void density_walk_halo(DensityWalk& w, uint32_t i, float4 xi,
         for (int u = 0; u < 4; ++u) {
           float dx = xi.x - c[u].x, dy2 = xi.y - c[u].y, dz2 = xi.z - c[u].z;
           w.visit((uint32_t)u < rem, dx * dx + dy2 * dy2 + dz2 * dz2, h2,
-                  __float_as_uint(c[u].w), i, nbr, n);
+                  __float_as_uint(c[u].w), i);
         }
       }
     }
@@ -373,18 +378,18 @@ __global__ __launch_bounds__(WAVES * 64) void k_density(const float4* __restrict
   uint32_t hc[3];
   float fr[3];
   half_cell_frac(xi, dp, hc, fr);
-  DensityWalk w;
+  DensityWalk w(nbr, i, n);
   if (ci < staged)
-    density_walk_halo(w, i, xi, hc, fr, sh.cand, sh.off + 64u * ci, h2, nbr, n);
+    density_walk_halo(w, i, xi, hc, fr, sh.cand, sh.off + 64u * ci, h2);
   else
-    density_walk_global(w, i, xi, hc, fr, dp, F, pos, table, h2, nbr, n);
+    density_walk_global(w, i, xi, hc, fr, dp, F, pos, table, h2, n);
   const uint32_t cnt = w.cnt, r = cnt & 3u;
   if (cnt < kMaxNeighbours && r) {
     // the r pending entries are p(4-r) .. p3, oldest first
     uint4 last = r == 1 ? make_uint4(w.p3, 0u, 0u, 0u)
                         : (r == 2 ? make_uint4(w.p2, w.p3, 0u, 0u)
                                   : make_uint4(w.p1, w.p2, w.p3, 0u));
-    nbr[(size_t)(cnt >> 2) * n + i] = last;
+    *w.row = last;  // row cnt / 4
   }
   nbr_count[i] = cnt;
   // density = sum m * poly6 * (h^2 - r^2)^3 ; Tait pressure (sph.cl:36-39)
