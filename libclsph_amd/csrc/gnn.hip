@@ -1,7 +1,9 @@
-// Kernels of the learned force surrogate (gfx950): a bf16 matrix-core GEMM
-// with float32 accumulation, the neighbour-graph and edge-input builders,
-// activation / aggregation / loss kernels and Adam. The forward and
-// backward passes are sequences of these launches on the context stream.
+// Kernels of the two-round message-passing network (gfx950): the
+// neighbour-graph and edge-input builders, the fused edge-round and node-MLP
+// inference kernels, activation / aggregation / loss kernels and Adam, with
+// the bf16 matrix-core GEMM of nn.hip for the training products. The
+// forward and backward passes are sequences of these launches on the
+// context stream.
 #include <hip/hip_runtime.h>
 
 #include "gnn.h"

@@ -32,7 +32,7 @@ void free_gnn(clsph_gnn* m) {
 int gnn_check_state(clsph_gnn* m, clsph_state* st) {
   if (!m || !st) return fail(m ? m->ctx : nullptr, CLSPH_EINVAL, "NULL argument");
   if (st->n_scenes != 1)
-    return fail(m->ctx, CLSPH_EINVAL, "the learned surrogate runs on single-scene states");
+    return fail(m->ctx, CLSPH_EINVAL, "the two-round network runs on single-scene states");
   if (st->n > m->n_cap)
     return fail(m->ctx, CLSPH_EINVAL, "state holds more particles than the network workspace");
   return CLSPH_OK;

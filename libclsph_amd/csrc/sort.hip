@@ -531,9 +531,6 @@ __global__ __launch_bounds__(kCellBlock) void k_cells(
   __shared__ uint32_t pre[kCellBlock / 64][9];   // the same before each wave, from a
   __shared__ uint32_t carry[9];                  // the same before the chunk, from a
   __shared__ uint32_t found;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint64_t lt = lanes_below(lane);
-  const uint64_t le = lt | (1ull << lane);
   const uint32_t total = bp->key_total;
   const bool table_ok = total <= bp->table_cap && !(bp->err & kErrFineKey);
   const uint32_t r0 = blockIdx.x * kCellRange;
