@@ -127,40 +127,68 @@ __device__ inline void for_each_run(const uint32_t hc[3], const float fr[3], uin
   }
 }
 
-// Density sum and neighbour list of one particle i. The list holds the
-// neighbours (r < h, self excluded) in visiting order, four to a uint4, at
-// nbr[q * n + i] (q < kMaxNeighbours / 4); pending entries wait in a shift
-// register whose newest entry is p3, and `row` points at the list row the
-// next four go to (nbr + i, then one row of n entries further per store).
+// Two- and four-lane unsigned vectors, held in aligned register pairs and
+// quads; the neighbour list's rows are addressed as global memory.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) u32x4 global_u32x4;
+
+// (p0, p1, p2, p3) <- (p1, p2, p3, tail.y): the quad as two register pairs,
+// each filled by one 64-bit move that takes one 32-bit half from each of two
+// pairs (dst.lo = src0 half op_sel[0], dst.hi = src1 half op_sel[1]).
+__device__ inline void shift_in(u32x4& reg, u32x2 tail) {
+  u32x2 lo = reg.xy, hi = reg.zw;
+  asm("v_pk_mov_b32 %0, %0, %1 op_sel:[1,0]" : "+v"(lo) : "v"(hi));
+  asm("v_pk_mov_b32 %0, %0, %1 op_sel:[1,1]" : "+v"(hi) : "v"(tail));
+  reg = (u32x4){lo.x, lo.y, hi.x, hi.y};
+}
+
+// Density sum and neighbour list of one particle i. The sum starts with i's
+// own term (r = 0: (h^2)^3) and adds the other candidates within h in
+// visiting order. The list holds those neighbours (self excluded) in
+// visiting order, four to a uint4, at nbr[q * n + i] (q < kMaxNeighbours / 4);
+// pending entries wait in the shift register `reg` (oldest in .x, newest in
+// .w), and `row` points at the list row the next four go to (nbr + i, then
+// one row of n entries further per store).
 struct DensityWalk {
-  float acc = 0.f;
-  uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0, cnt = 0;
-  uint4* row;
-  uint32_t n;
+  float acc;
+  u32x4 reg = {0u, 0u, 0u, 0u};
+  uint32_t cnt = 0;
+  global_u32x4* row;
+  uint64_t n;
 
-  __device__ DensityWalk(uint4* __restrict__ nbr, uint32_t i, uint32_t n_) : row(nbr + i), n(n_) {}
+  __device__ DensityWalk(uint4* __restrict__ nbr, uint32_t i, uint32_t n_, float h2)
+      : acc((h2 * h2) * h2), row((global_u32x4*)(nbr + i)), n(n_) {}
 
-  // Candidate j at squared distance d2; in_run is false for the padding
-  // entries of a group of four.
-  __device__ inline void visit(bool in_run, float d2, float h2, uint32_t j, uint32_t i) {
+  // Candidate j = tail.y at squared distance d2; in_run is false for the
+  // padding entries of a group of four. Testing j != i with the distance
+  // keeps the append below unconditional for the lanes that take it.
+  __device__ inline void visit(bool in_run, float d2, float h2, u32x2 tail, uint32_t i) {
     static_assert((kMaxNeighbours & (kMaxNeighbours - 1u)) == 0u && kMaxNeighbours >= 4u,
                   "the list-room test below needs a power of two");
-    if (in_run && d2 < h2) {
+    if (in_run && d2 < h2 && tail.y != i) {
       float t = h2 - d2;
       acc += t * t * t;
-      const bool app = j != i;
       // the register shifts on every neighbour and is stored after each
       // fourth one while the list has room: cnt % 4 == 3 and cnt < kMaxNeighbours
-      p0 = app ? p1 : p0;
-      p1 = app ? p2 : p1;
-      p2 = app ? p3 : p2;
-      p3 = app ? j : p3;
-      if (app && (cnt & (3u - kMaxNeighbours)) == 3u) {
-        *row = make_uint4(p0, p1, p2, p3);
-        row += n;
+      const bool full = (cnt & (3u - kMaxNeighbours)) == 3u;
+      shift_in(reg, tail);
+      if (full) {
+        *row = reg;
+        // the next row, after the store has read `row`, in place
+        asm volatile("v_lshl_add_u64 %0, %1, 4, %0" : "+v"(row) : "s"(n) : "memory");
       }
-      cnt += app ? 1u : 0u;
+      ++cnt;
     }
+  }
+
+  // Entries still pending (cnt % 4 of them, the newest of the register) go
+  // to row cnt / 4 while the list has room.
+  __device__ inline void finish() {
+    const uint32_t r = cnt & 3u;
+    if (cnt < kMaxNeighbours && r)
+      *row = r == 1 ? (u32x4){reg.w, 0u, 0u, 0u}
+                    : (r == 2 ? (u32x4){reg.z, reg.w, 0u, 0u} : (u32x4){reg.y, reg.z, reg.w, 0u});
   }
 };
 
@@ -184,7 +212,7 @@ __device__ inline void density_walk_global(DensityWalk& w, uint32_t i, float4 xi
       for (int u = 0; u < 4; ++u) {
         const uint32_t j = j0 + u;
         float dx = xi.x - c[u].x, dy = xi.y - c[u].y, dz = xi.z - c[u].z;
-        w.visit(j < e, dx * dx + dy * dy + dz * dz, h2, j, i);
+        w.visit(j < e, dx * dx + dy * dy + dz * dz, h2, (u32x2){j, j}, i);
       }
     }
   });
@@ -219,8 +247,9 @@ __device__ inline void density_walk_halo(DensityWalk& w, uint32_t i, float4 xi,
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           float dx = xi.x - c[u].x, dy2 = xi.y - c[u].y, dz2 = xi.z - c[u].z;
+          // (z, index): the candidate's last register pair
           w.visit((uint32_t)u < rem, dx * dx + dy2 * dy2 + dz2 * dz2, h2,
-                  __float_as_uint(c[u].w), i);
+                  (u32x2){__float_as_uint(c[u].z), __float_as_uint(c[u].w)}, i);
         }
       }
     }
@@ -378,20 +407,13 @@ __global__ __launch_bounds__(WAVES * 64) void k_density(const float4* __restrict
   uint32_t hc[3];
   float fr[3];
   half_cell_frac(xi, dp, hc, fr);
-  DensityWalk w(nbr, i, n);
+  DensityWalk w(nbr, i, n, h2);
   if (ci < staged)
     density_walk_halo(w, i, xi, hc, fr, sh.cand, sh.off + 64u * ci, h2);
   else
     density_walk_global(w, i, xi, hc, fr, dp, F, pos, table, h2, n);
-  const uint32_t cnt = w.cnt, r = cnt & 3u;
-  if (cnt < kMaxNeighbours && r) {
-    // the r pending entries are p(4-r) .. p3, oldest first
-    uint4 last = r == 1 ? make_uint4(w.p3, 0u, 0u, 0u)
-                        : (r == 2 ? make_uint4(w.p2, w.p3, 0u, 0u)
-                                  : make_uint4(w.p1, w.p2, w.p3, 0u));
-    *w.row = last;  // row cnt / 4
-  }
-  nbr_count[i] = cnt;
+  w.finish();
+  nbr_count[i] = w.cnt;
   // density = sum m * poly6 * (h^2 - r^2)^3 ; Tait pressure (sph.cl:36-39)
   float m = dp->p.particle_mass;
   float d = m * (dp->t.poly_6 * w.acc);
