@@ -143,6 +143,14 @@ __device__ inline void shift_in(u32x4& reg, u32x2 tail) {
   reg = (u32x4){lo.x, lo.y, hi.x, hi.y};
 }
 
+// Squared distance of a candidate, with every rounding written out so that
+// both walks (and any inlining of them) compute the same bits.
+__device__ inline float dist2(float4 xi, float4 c) {
+#pragma clang fp contract(off)
+  const float dx = xi.x - c.x, dy = xi.y - c.y, dz = xi.z - c.z;
+  return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
 // Density sum and neighbour list of one particle i. The sum starts with i's
 // own term (r = 0: (h^2)^3) and adds the other candidates within h in
 // visiting order. The list holds those neighbours (self excluded) in
@@ -167,8 +175,8 @@ struct DensityWalk {
     static_assert((kMaxNeighbours & (kMaxNeighbours - 1u)) == 0u && kMaxNeighbours >= 4u,
                   "the list-room test below needs a power of two");
     if (in_run && d2 < h2 && tail.y != i) {
-      float t = h2 - d2;
-      acc += t * t * t;
+      const float t = h2 - d2;
+      acc = __builtin_fmaf(t, t * t, acc);
       // the register shifts on every neighbour and is stored after each
       // fourth one while the list has room: cnt % 4 == 3 and cnt < kMaxNeighbours
       const bool full = (cnt & (3u - kMaxNeighbours)) == 3u;
@@ -211,8 +219,7 @@ __device__ inline void density_walk_global(DensityWalk& w, uint32_t i, float4 xi
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint32_t j = j0 + u;
-        float dx = xi.x - c[u].x, dy = xi.y - c[u].y, dz = xi.z - c[u].z;
-        w.visit(j < e, dx * dx + dy * dy + dz * dz, h2, (u32x2){j, j}, i);
+        w.visit(j < e, dist2(xi, c[u]), h2, (u32x2){j, j}, i);
       }
     }
   });
@@ -238,17 +245,19 @@ __device__ inline void density_walk_halo(DensityWalk& w, uint32_t i, float4 xi,
       const uint32_t row = 4u * (ly + (uint32_t)dy) + 16u * (lz + (uint32_t)dz);
       const uint32_t xa = lo_x + ryz < kReach2 ? lx - 1u : lx;
       const uint32_t xb = hi_x + ryz < kReach2 ? lx + 1u : lx;
-      const uint32_t s = off[row + xa], e = off[row + xb + 1u];
-      for (uint32_t j0 = s; j0 < e; j0 += 4) {
+      // byte offsets into cand: the group at a holds entry u inside the
+      // range when a < end[u] = 16 (e - u)
+      const int s = (int)off[row + xa] * 16, e = (int)off[row + xb + 1u] * 16;
+      const int end[4] = {e, e - 16, e - 32, e - 48};
+      for (int a = s; a < e; a += 64) {
+        const float4* g = (const float4*)((const char*)cand + a);
         float4 c[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) c[u] = cand[j0 + u];
-        const uint32_t rem = e - j0;  // entries of the group inside the range (>= 1)
+        for (int u = 0; u < 4; ++u) c[u] = g[u];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          float dx = xi.x - c[u].x, dy2 = xi.y - c[u].y, dz2 = xi.z - c[u].z;
           // (z, index): the candidate's last register pair
-          w.visit((uint32_t)u < rem, dx * dx + dy2 * dy2 + dz2 * dz2, h2,
+          w.visit(u == 0 || a < end[u], dist2(xi, c[u]), h2,
                   (u32x2){__float_as_uint(c[u].z), __float_as_uint(c[u].w)}, i);
         }
       }
