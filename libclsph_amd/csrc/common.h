@@ -121,11 +121,32 @@ __device__ inline float2 node_in(float rho, float p, float rho0, float K) {
   return make_float2(rho / rho0 - 1.f, asinhf(p / K) * 0.25f);
 }
 
-// The particle a thread of a per-particle kernel handles, or UINT32_MAX when
-// the thread is past the end of its scene.
+// The chip deals a launch's work-groups to its kXcds XCDs round-robin by
+// index (XCD = blockIdx.x % kXcds), and each XCD has an L2 of its own. This
+// maps launch index b of G to an entry of the work list such that each XCD
+// takes runs of SPAN consecutive entries: launch indices
+// [g, g + kXcds SPAN) take the list's entries of that range, XCD x the x-th
+// run of them (a last partial range keeps the launch order). Work-groups
+// that gather each other's particles then share an L2, while every XCD still
+// samples the whole list (one contiguous share per XCD unbalances them:
+// particles' work differs across a scene).
+constexpr uint32_t kXcds = 8;
+template <uint32_t SPAN>
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t G) {
+  constexpr uint32_t group = kXcds * SPAN;
+  const uint32_t g0 = b - b % group;
+  if (g0 + group > G) return b;
+  const uint32_t w = b - g0;
+  return g0 + (w % kXcds) * SPAN + w / kXcds;
+}
+
+// The particle of this thread (0xFFFFFFFF past its scene's end) and its
+// scene, from the launch's work list; SPAN > 0: the list taken in
+// xcd_block<SPAN> order.
+template <uint32_t SPAN = 0>
 __device__ inline uint32_t particle_of(const BlockDesc* __restrict__ blocks,
                                        const DevParams* __restrict__ dps, uint32_t& scene) {
-  BlockDesc b = blocks[blockIdx.x];
+  BlockDesc b = blocks[SPAN ? xcd_block<SPAN>(blockIdx.x, gridDim.x) : blockIdx.x];
   scene = b.scene;
   uint32_t i = b.first + threadIdx.x;
   const DevParams& d = dps[b.scene];

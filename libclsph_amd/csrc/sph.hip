@@ -321,7 +321,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_density(const float4* __restrict
   using Halos = DensityHalos<WAVES>;
   static_assert(Halos::kHalos <= 2u * WAVES, "at most two (halo, half-cell) entries per thread");
   __shared__ Halos sh;
-  const BlockDesc b = blocks[blockIdx.x];
+  // runs of 16 work-groups per XCD: neighbouring halos share an L2
+  const BlockDesc b = blocks[xcd_block<16>(blockIdx.x, gridDim.x)];
   const DevParams* dp = dps + b.scene;
   const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
   const uint32_t se = dp->begin + dp->n;
@@ -490,7 +491,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_forces(const float4* __restrict__
                                                       const BatchParams* __restrict__ bp,
                                                       float4* __restrict__ acc_out, uint32_t n) {
   uint32_t scene;
-  uint32_t i = particle_of(blocks, dps, scene);
+  // runs of 4 work-groups per XCD: neighbour records shared in an L2
+  uint32_t i = particle_of<4>(blocks, dps, scene);
   if (i == 0xFFFFFFFFu) return;
   if (OVERFLOW_ONLY && nbr_count[i] <= kMaxNeighbours) return;
   const DevParams* dp = dps + scene;
