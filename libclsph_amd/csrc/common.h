@@ -67,10 +67,17 @@ struct BatchParams {
   uint32_t table_cap;    // half-cell table capacity in keys
   uint32_t key_bits;     // bitlen(key_total)
   uint32_t sort_passes;  // LSD passes over the cell-id bits (set by the host)
-  uint32_t digit_bits;   // bits per pass: ceil((key_bits - 3) / sort_passes), at most 11
+  uint32_t digit_bits;   // bits per pass: ceil((key_bits - 3) / lsd_passes), at most 11
   uint32_t err;
   uint32_t ticket;       // grid-parameter work-groups done this substep (zero between)
+  uint32_t lsd_passes;   // LSD passes run this substep: sort_passes, or 2 when the cell
+                         // ids fit two (the graph's third pass then does nothing)
 };
+
+// LSD passes of this substep (sort_passes before the first layout).
+__device__ inline uint32_t lsd_passes_of(const BatchParams* __restrict__ bp) {
+  return bp->lsd_passes ? bp->lsd_passes : bp->sort_passes;
+}
 
 // Work-group descriptor for per-particle kernels: scene and first particle.
 struct BlockDesc {

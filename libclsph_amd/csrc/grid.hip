@@ -39,9 +39,12 @@ __device__ void batch_layout(BatchParams* __restrict__ bp, uint64_t base, uint32
   bp->key_total = (uint32_t)base;
   uint32_t kb = bitlen((uint32_t)base);
   bp->key_bits = kb;
-  // the sort orders by cell id (key >> 3) in sort_passes LSD passes
+  // the sort orders by cell id (key >> 3) in sort_passes LSD passes; a state
+  // sized for three runs two while the cell ids fit them
   uint32_t cb = kb > 3u ? kb - 3u : 1u;
   uint32_t passes = sort_passes ? sort_passes : 1u;
+  if (passes > 2u && cb <= 2u * (uint32_t)kMaxDigitBits) passes = 2u;
+  bp->lsd_passes = passes;
   uint32_t db = (cb + passes - 1u) / passes;
   bp->digit_bits = db < 1u ? 1u : (db > (uint32_t)kMaxDigitBits ? (uint32_t)kMaxDigitBits : db);
   if (err) atomicOr(&bp->err, err);

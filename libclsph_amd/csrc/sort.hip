@@ -13,7 +13,10 @@
 //     k_lsd_scatter (keys ranked in registers with wave64 ballot matching,
 //     the tile reordered digit-major in LDS, each digit run written to
 //     consecutive addresses). After the passes the particles are in the
-//     reference's order, with their canonical indices.
+//     reference's order, with their canonical indices. A state whose table
+//     capacity needs three passes launches three, and its third pass does
+//     nothing in a substep whose cell ids fit two (22 bits): the pass count
+//     of the substep is set on the device with the digit width.
 //  2. k_cells: each cell's particles are contiguous in that order; a
 //     work-group takes the cells starting in its 256 positions and orders
 //     each cell by octant (stably), which gives the sorted (half-cell)
@@ -110,6 +113,7 @@ __global__ __launch_bounds__(kLsdBlock) void k_lsd_count(const uint32_t* __restr
   __shared__ uint32_t bins[kMaxBins];
   // the first pass also empties the cell pass's list of long runs
   if (pass == 0 && blockIdx.x == 0 && threadIdx.x == 0) runs_ctl[0] = 0;
+  if (pass >= lsd_passes_of(bp)) return;
   const uint32_t db = bp->digit_bits, nb = 1u << db;
   for (uint32_t d = threadIdx.x; d < nb; d += kLsdBlock) bins[d] = 0;
   const int lane = threadIdx.x & 63;
@@ -147,11 +151,12 @@ __global__ __launch_bounds__(kLsdBlock) void k_lsd_count(const uint32_t* __restr
 __global__ __launch_bounds__(kScanBlock) void k_lsd_scan(uint32_t* __restrict__ hist,
                                                          uint32_t n_tiles,
                                                          const BatchParams* __restrict__ bp,
+                                                         uint32_t pass,
                                                          uint32_t* __restrict__ totals) {
   constexpr uint32_t kDigits = 16, kSlices = kScanBlock / kDigits;
   __shared__ uint32_t part[kSlices][kDigits];
   const uint32_t nb = 1u << bp->digit_bits;
-  if (blockIdx.x * kDigits >= nb) return;
+  if (blockIdx.x * kDigits >= nb || pass >= lsd_passes_of(bp)) return;
   const uint32_t dl = threadIdx.x % kDigits, g = threadIdx.x / kDigits;
   const uint32_t d = blockIdx.x * kDigits + dl;
   const bool on = d < nb;
@@ -196,6 +201,7 @@ __global__ __launch_bounds__(kLsdBlock) void k_lsd_scatter(
   __shared__ uint32_t dbase[kMaxBins];            // output start of the tile's run of each digit
   __shared__ uint32_t skey[kLsdTile];
   __shared__ uint32_t sval[kLsdTile];
+  if (pass >= lsd_passes_of(bp)) return;
   const uint32_t db = bp->digit_bits, nb = 1u << db;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t base = blockIdx.x * kLsdTile + w * kLsdWaveKeys;
@@ -513,7 +519,8 @@ __device__ __forceinline__ void cells_place(
 }
 
 __global__ __launch_bounds__(kCellBlock) void k_cells(
-    const uint32_t* __restrict__ sk, const uint32_t* __restrict__ sv, uint32_t n,
+    const uint32_t* __restrict__ sk_odd, const uint32_t* __restrict__ sv_odd,
+    const uint32_t* __restrict__ sk_even, const uint32_t* __restrict__ sv_even, uint32_t n,
     const BatchParams* __restrict__ bp, uint4* __restrict__ runs, uint32_t* __restrict__ runs_ctl,
     uint32_t runs_cap, uint32_t* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out, uint32_t* __restrict__ slot, uint32_t* __restrict__ table,
@@ -531,6 +538,10 @@ __global__ __launch_bounds__(kCellBlock) void k_cells(
   __shared__ uint32_t pre[kCellBlock / 64][9];   // the same before each wave, from a
   __shared__ uint32_t carry[9];                  // the same before the chunk, from a
   __shared__ uint32_t found;
+  // the last LSD pass's output: (key_t, val_t) after an odd number of passes
+  const bool odd = (lsd_passes_of(bp) & 1u) != 0u;
+  const uint32_t* __restrict__ sk = odd ? sk_odd : sk_even;
+  const uint32_t* __restrict__ sv = odd ? sv_odd : sv_even;
   const uint32_t total = bp->key_total;
   const bool table_ok = total <= bp->table_cap && !(bp->err & kErrFineKey);
   const uint32_t r0 = blockIdx.x * kCellRange;
@@ -621,10 +632,8 @@ __global__ __launch_bounds__(256) void k_fill_runs(const uint4* __restrict__ run
 __global__ void k_clear_runs(uint32_t* __restrict__ runs_ctl) { runs_ctl[0] = 0; }
 
 void cell_pass(clsph_state* s) {
-  const uint32_t* sk = (s->sort_passes & 1) ? s->key_t : s->key_u;
-  const uint32_t* sv = (s->sort_passes & 1) ? s->val_t : s->val_u;
   hipLaunchKernelGGL(k_cells, dim3((s->n + kCellRange - 1) / kCellRange), dim3(kCellBlock), 0,
-                     s->ctx->stream, sk, sv, s->n, s->bp, s->sort_runs, s->sort_runs_ctl,
+                     s->ctx->stream, s->key_t, s->val_t, s->key_u, s->val_u, s->n, s->bp, s->sort_runs, s->sort_runs_ctl,
                      s->sort_runs_cap, s->key_b, s->val_b, s->slot, s->table, s->pos, s->vel,
                      s->vh, s->pos_s, s->vel_s, s->vh_s);
   hipLaunchKernelGGL(k_fill_runs, dim3(kFillBlocks), dim3(256), 0, s->ctx->stream, s->sort_runs,
@@ -653,7 +662,7 @@ void launch_sort(clsph_state* s) {
     hipLaunchKernelGGL(k_lsd_count, dim3(tiles), dim3(kLsdBlock), 0, st, kin, s->n, s->bp, p,
                        s->sort_hist, s->sort_runs_ctl);
     hipLaunchKernelGGL(k_lsd_scan, dim3(kMaxBins / 16), dim3(kScanBlock), 0, st, s->sort_hist,
-                       tiles, s->bp, s->sort_totals);
+                       tiles, s->bp, p, s->sort_totals);
     hipLaunchKernelGGL(k_lsd_scatter, dim3(tiles), dim3(kLsdBlock), 0, st, kin, vin, s->n, s->bp,
                        p, s->sort_hist, s->sort_totals, kout, vout);
     kin = kout;
