@@ -417,14 +417,24 @@ __device__ __forceinline__ void cells_place(
   }
   __syncthreads();
 
-  // table, written in key order over the work-group's keys: from its first
-  // cell (from 0 for the first work-group) up to the next work-group's first
-  // cell (through key_total for the last one), a cell and the empty keys
-  // after it at a time. A key of an owned cell holds the start of its octant
-  // run; an empty key the start of the next cell. Runs of more than
+  // table over the work-group's keys: from its first cell (from 0 for the
+  // first work-group) up to the next work-group's first cell (through
+  // key_total for the last one), the owned cells' keys first, then the empty
+  // keys after each cell. A key of an owned cell holds the start of its
+  // octant run; an empty key the start of the next cell. Runs of more than
   // kLongRun empty keys (the sparse Morton range leaves a few of hundreds of
   // thousands) are listed for k_fill_runs, which spreads them over the chip.
   if (table_ok) {
+    // the eight keys of every owned cell, one (cell, octant) per thread: the
+    // cell's start plus its particles of lower octants
+    for (uint32_t e = threadIdx.x; e < 8u * ncells; e += kCellBlock) {
+      const uint32_t j = e >> 3, q = e & 7u;
+      uint32_t r = c_start[j];
+#pragma unroll
+      for (uint32_t p = 0; p < 7u; ++p) r += p < q ? c_oct[j + 1][p] - c_oct[j][p] : 0u;
+      table[(c_id[j] << 3) + q] = r;
+    }
+    // the empty keys before the first cell and after each cell
     const uint32_t kend = b < n ? (KEY_AT(b) & ~7u) : total + 1u;
     for (int j = -1; j < (int)ncells; ++j) {
       uint32_t f0, f1, v;
@@ -432,13 +442,7 @@ __device__ __forceinline__ void cells_place(
         if (a != 0) continue;  // the keys before the first cell: the first work-group's
         f0 = 0, f1 = c_id[0] << 3, v = 0;
       } else {
-        const uint32_t c8 = c_id[j] << 3;
-        if (threadIdx.x < 8) {
-          uint32_t r = c_start[j];
-          for (uint32_t q = 0; q < threadIdx.x; ++q) r += c_oct[j + 1][q] - c_oct[j][q];
-          table[c8 + threadIdx.x] = r;
-        }
-        f0 = c8 + 8u;
+        f0 = (c_id[j] << 3) + 8u;
         f1 = j + 1 < (int)ncells ? (c_id[j + 1] << 3) : kend;
         v = c_start[j + 1];
       }
