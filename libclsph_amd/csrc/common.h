@@ -153,7 +153,9 @@ __device__ inline uint32_t xcd_block(uint32_t b, uint32_t G) {
 template <uint32_t SPAN = 0>
 __device__ inline uint32_t particle_of(const BlockDesc* __restrict__ blocks,
                                        const DevParams* __restrict__ dps, uint32_t& scene) {
-  BlockDesc b = blocks[SPAN ? xcd_block<SPAN>(blockIdx.x, gridDim.x) : blockIdx.x];
+  uint32_t e = blockIdx.x;
+  if constexpr (SPAN != 0) e = xcd_block<SPAN>(blockIdx.x, gridDim.x);
+  BlockDesc b = blocks[e];
   scene = b.scene;
   uint32_t i = b.first + threadIdx.x;
   const DevParams& d = dps[b.scene];
