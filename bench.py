@@ -461,26 +461,39 @@ def bench_batched(args):
     b.set_block_sizes(sizes)
     print(f"# device {ctx.device_name()}; 64 scenes, {n} particles; work-group sizes {sizes}",
           file=sys.stderr, flush=True)
-    if warmup:
-        b.step(warmup)
-    ctx.sync()
-    ms = b.time_steps(steps)
+    if args.direct:
+        # every substep launched stage by stage, HIP events between stages
+        if warmup:
+            b.profile_steps(warmup)
+        ctx.sync()
+        t0 = time.perf_counter()
+        prof = b.profile_steps(steps)
+        ctx.sync()
+        ms = (time.perf_counter() - t0) * 1e3
+    else:
+        if warmup:
+            b.step(warmup)
+        ctx.sync()
+        ms = b.time_steps(steps)
+        prof = b.profile_steps(args.profile_steps if args.profile_steps > 0 else 50)
     ms_step = ms / steps
-    prof = b.profile_steps(args.profile_steps if args.profile_steps > 0 else 50)
     finite = all(bool(np.isfinite(b.download_scene(i)["position"][:, :3]).all())
                  for i in range(0, 64, 7))
     b.close()
-    # The same scenes one after another, each in its own single-scene state.
-    seq = []
-    for sc in batch_scenes:
-        st = native.State(ctx, sc.n)
-        st.set_params(sc.params, sc.terms)
-        st.scene(sc.normals, sc.vertices, sc.indices)
-        st.upload(sc.particles)
-        st.step(warmup)
-        seq.append(st.time_steps(20) / 20)
-        st.close()
-    seq_ms = float(sum(seq))
+    # The same scenes one after another, each in its own single-scene state
+    # (graph replay; not run with --direct, whose launches are not comparable).
+    seq_ms = None
+    if not args.direct:
+        seq = []
+        for sc in batch_scenes:
+            st = native.State(ctx, sc.n)
+            st.set_params(sc.params, sc.terms)
+            st.scene(sc.normals, sc.vertices, sc.indices)
+            st.upload(sc.particles)
+            st.step(warmup)
+            seq.append(st.time_steps(20) / 20)
+            st.close()
+        seq_ms = float(sum(seq))
     nbytes = stage_bytes(n, 0)
     dom = max((k for k in prof if k != "grid"), key=lambda k: prof[k])
     achieved = nbytes[dom] / (prof[dom] * 1e-3) / 1e9
@@ -499,10 +512,11 @@ def bench_batched(args):
         "data": "synthetic",
         "config": {"workload": "batched_dam_break", "scenes": 64, "particles": n,
                    "min_scene": min(counts), "max_scene": max(counts),
-                   "block_sizes": sizes, "state_finite": finite},
+                   "block_sizes": sizes, "state_finite": finite,
+                   "launch": "stage by stage" if args.direct else "graph replay"},
         "per_scene_latency_ms": ms_step,
         "sequential_ms_per_step": seq_ms,
-        "batching_speedup": seq_ms / ms_step,
+        "batching_speedup": seq_ms / ms_step if seq_ms else None,
         "stages_ms": {k: round(v, 5) for k, v in prof.items()},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},

@@ -51,22 +51,39 @@ __device__ void batch_layout(BatchParams* __restrict__ bp, uint64_t base, uint32
 }
 
 // Lays the scenes' half-cell key ranges end to end (each scene gets
-// 8 (G + 1) keys) and sizes the sort digits for the total (one thread).
+// 8 (G + 1) keys, its base clamped at 0xFFFFFFF0 with kErrFineKey raised
+// once the running total passes it) and sizes the sort digits for the total.
+// One wave, 64 scenes at a time: each scene's base is the exclusive sum of
+// the keys before it, a wave scan.
 __device__ void batch_keys(DevParams* __restrict__ dps, BatchParams* __restrict__ bp) {
-  uint64_t base = 0;
-  uint32_t err = 0;
-  for (uint32_t s = 0; s < bp->n_scenes; ++s) {
-    DevParams* dp = dps + s;
-    uint64_t G = (dp->err & kErrFineKey) ? 0u : dp->p.grid_cell_count;
-    dp->key_base = (uint32_t)base;
-    dp->key_end = (uint32_t)(base + 8ull * G);
-    base += 8ull * (G + 1);
-    if (base > 0xFFFFFFF0ull) {
-      err |= kErrFineKey;
-      base = 0xFFFFFFF0ull;
+  constexpr uint64_t kTop = 0xFFFFFFF0ull;
+  const uint32_t lane = threadIdx.x & 63u, ns = bp->n_scenes;
+  uint64_t carry = 0;
+  for (uint32_t s0 = 0; s0 < ns; s0 += 64u) {
+    const uint32_t s = s0 + lane;
+    uint64_t G = 0, keys = 0;
+    if (s < ns) {
+      G = (dps[s].err & kErrFineKey) ? 0u : dps[s].p.grid_cell_count;
+      keys = 8ull * (G + 1);
     }
+    uint64_t incl = keys;
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+      const uint32_t lo = __shfl_up((uint32_t)incl, o, 64), hi = __shfl_up((uint32_t)(incl >> 32), o, 64);
+      if (lane >= o) incl += ((uint64_t)hi << 32) | lo;
+    }
+    if (s < ns) {
+      const uint64_t ex = carry + incl - keys;
+      const uint64_t base = ex < kTop ? ex : kTop;
+      dps[s].key_base = (uint32_t)base;
+      dps[s].key_end = (uint32_t)(base + 8ull * G);
+    }
+    const uint32_t tlo = __shfl((uint32_t)incl, 63, 64), thi = __shfl((uint32_t)(incl >> 32), 63, 64);
+    carry += ((uint64_t)thi << 32) | tlo;
   }
-  batch_layout(bp, base, err, bp->table_cap, bp->sort_passes);
+  if (lane == 0)
+    batch_layout(bp, carry < kTop ? carry : kTop, carry > kTop ? kErrFineKey : 0u, bp->table_cap,
+                 bp->sort_passes);
 }
 
 // One work-group per scene: reduces the scene's partial boxes, then restates
@@ -109,15 +126,18 @@ __global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dp
   if (l == 0)
     for (int k = 0; k < 6; ++k) red[k][w] = v[k];
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  int nw = blockDim.x >> 6;
-  for (int k = 0; k < 6; ++k) {
-    float r = red[k][0];
-    for (int q = 1; q < nw; ++q) r = (k < 3) ? fminf(r, red[k][q]) : fmaxf(r, red[k][q]);
-    v[k] = r;
+  if (gridDim.x > 1u && threadIdx.x >= 64u) return;  // the first wave may lay out the keys
+  if (threadIdx.x == 0) {
+    int nw = blockDim.x >> 6;
+    for (int k = 0; k < 6; ++k) {
+      float r = red[k][0];
+      for (int q = 1; q < nw; ++q) r = (k < 3) ? fminf(r, red[k][q]) : fmaxf(r, red[k][q]);
+      v[k] = r;
+    }
   }
-  uint32_t G;
-  const uint32_t err = scene_grid(dp, v, h, np, G);
+  if (threadIdx.x != 0 && gridDim.x == 1u) return;
+  uint32_t G = 0, err = 0;
+  if (threadIdx.x == 0) err = scene_grid(dp, v, h, np, G);
   if (gridDim.x == 1u) {
     // one scene: batch_keys with the scene's values in registers
     const uint64_t Gk = ((err0 | err) & kErrFineKey) ? 0u : G;
@@ -133,12 +153,15 @@ __global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dp
     return;
   }
   // the scenes' grids are in place once every work-group has passed here
-  __threadfence();
-  if (atomicAdd(&bp->ticket, 1u) == gridDim.x - 1u) {
+  uint32_t last = 0;
+  if (threadIdx.x == 0) {
     __threadfence();
-    bp->ticket = 0;
-    batch_keys(dps, bp);
+    last = atomicAdd(&bp->ticket, 1u) == gridDim.x - 1u ? 1u : 0u;
   }
+  if (!__shfl(last, 0, 64)) return;
+  __threadfence();
+  if (threadIdx.x == 0) bp->ticket = 0;
+  batch_keys(dps, bp);
 }
 
 // The grid of one scene from its bounding box v = (min xyz, max xyz), its
