@@ -144,6 +144,8 @@ def sparse_cloud(n, seed, span=11.0):
 THREE_PASS_CASES = {
     "dam4096_two_of_three": (lambda: scenes.dam_break(4096, seed=7), 22),
     "sparse3000_three_of_three": (lambda: sparse_cloud(3000, 11), 23),
+    # three passes over 293 tiles of 2,048 keys
+    "sparse600k_three_of_three": (lambda: sparse_cloud(600000, 13), 23),
 }
 
 
