@@ -104,18 +104,18 @@ __global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dp
   const uint32_t table_cap = bp->table_cap, sort_passes = bp->sort_passes;
   const float inf = __builtin_huge_valf();
   float v[6] = {inf, inf, inf, -inf, -inf, -inf};
-  // four partial boxes per thread in flight at a time
+  // eight partial boxes per thread in flight at a time
   const uint32_t b0 = dp->block_begin, b1 = dp->block_end;
-  for (uint32_t b = b0 + threadIdx.x; b < b1; b += 4u * blockDim.x) {
-    float q[4][6];
+  for (uint32_t b = b0 + threadIdx.x; b < b1; b += 8u * blockDim.x) {
+    float q[8][6];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       const uint32_t bu = b + (uint32_t)u * blockDim.x;
 #pragma unroll
       for (int k = 0; k < 6; ++k) q[u][k] = bu < b1 ? partial[bu * 6 + k] : (k < 3 ? inf : -inf);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       for (int k = 0; k < 3; ++k) v[k] = fminf(v[k], q[u][k]);
       for (int k = 3; k < 6; ++k) v[k] = fmaxf(v[k], q[u][k]);
     }
@@ -126,14 +126,19 @@ __global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dp
   if (l == 0)
     for (int k = 0; k < 6; ++k) red[k][w] = v[k];
   __syncthreads();
-  if (gridDim.x > 1u && threadIdx.x >= 64u) return;  // the first wave may lay out the keys
-  if (threadIdx.x == 0) {
-    int nw = blockDim.x >> 6;
-    for (int k = 0; k < 6; ++k) {
-      float r = red[k][0];
-      for (int q = 1; q < nw; ++q) r = (k < 3) ? fminf(r, red[k][q]) : fmaxf(r, red[k][q]);
-      v[k] = r;
+  if (threadIdx.x >= 64u) return;  // the first wave joins the waves' boxes
+  // lanes 0-15 take one wave's box each; a butterfly over 16 lanes leaves
+  // the scene's box in every lane
+  static_assert(sizeof(red[0]) / sizeof(float) == 16, "16 waves per work-group");
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    float r = red[k][l & 15];
+#pragma unroll
+    for (int o = 8; o >= 1; o >>= 1) {
+      const float y = __shfl_xor(r, o, 64);
+      r = (k < 3) ? fminf(r, y) : fmaxf(r, y);
     }
+    v[k] = r;
   }
   if (threadIdx.x != 0 && gridDim.x == 1u) return;
   uint32_t G = 0, err = 0;

@@ -73,28 +73,6 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v) {
   return off + x - v;
 }
 
-// Exclusive scan, in place, of the first `count` (a multiple of B, at most
-// B * 8) entries of an LDS array.
-template <int B>
-__device__ inline void lds_exclusive_scan(uint32_t* a, uint32_t count) {
-  const uint32_t per = count / B;
-  const uint32_t b0 = threadIdx.x * per;
-  uint32_t v[8];
-  uint32_t local = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < 8; ++k) {
-    v[k] = k < per ? a[b0 + k] : 0u;
-    local += v[k];
-  }
-  uint32_t run = block_exclusive_scan<B>(local);
-#pragma unroll
-  for (uint32_t k = 0; k < 8; ++k) {
-    if (k < per) a[b0 + k] = run;
-    run += v[k];
-  }
-  __syncthreads();
-}
-
 // Digit of pass `pass` of a half-cell key: bits [3 + db * pass, +db), the
 // cell id's.
 __device__ inline uint32_t pass_digit(uint32_t key, uint32_t db, uint32_t pass) {
@@ -147,7 +125,11 @@ __global__ __launch_bounds__(kLsdBlock) void k_lsd_count(const uint32_t* __restr
 // Each digit's counts scanned over the tiles, in place (exclusive), and the
 // digit totals. A work-group takes 16 consecutive digits and cuts the tiles
 // into 64 slices, one thread per (slice, digit): sums per slice, joined
-// through LDS, then the running sums written back.
+// through LDS, then the running sums written back. REGS > 0: a slice has at
+// most REGS tiles, whose counts are all loaded at once and kept in registers
+// for the write-back (one round trip to memory each way); REGS = 0: any
+// number of tiles, loaded twice.
+template <uint32_t REGS>
 __global__ __launch_bounds__(kScanBlock) void k_lsd_scan(uint32_t* __restrict__ hist,
                                                          uint32_t n_tiles,
                                                          const BatchParams* __restrict__ bp,
@@ -162,26 +144,45 @@ __global__ __launch_bounds__(kScanBlock) void k_lsd_scan(uint32_t* __restrict__ 
   const bool on = d < nb;
   const uint32_t per = (n_tiles + kSlices - 1) / kSlices;
   const uint32_t t0 = min(g * per, n_tiles), t1 = min(t0 + per, n_tiles);
+  uint32_t* col = hist + d;
+  constexpr uint32_t R = REGS ? REGS : 1u;
+  uint32_t c[R];
   uint32_t sum = 0;
   if (on) {
+    if constexpr (REGS != 0) {
+#pragma unroll
+      for (uint32_t k = 0; k < R; ++k) {
+        c[k] = t0 + k < t1 ? col[(size_t)(t0 + k) * nb] : 0u;
+        sum += c[k];
+      }
+    } else {
 #pragma unroll 8
-    for (uint32_t t = t0; t < t1; ++t) sum += hist[(size_t)t * nb + d];
+      for (uint32_t t = t0; t < t1; ++t) sum += col[(size_t)t * nb];
+    }
   }
   part[g][dl] = sum;
   __syncthreads();
   uint32_t run = 0, all = 0;
   for (uint32_t q = 0; q < kSlices; ++q) {
-    const uint32_t c = part[q][dl];
-    run += q < g ? c : 0u;
-    all += c;
+    const uint32_t v = part[q][dl];
+    run += q < g ? v : 0u;
+    all += v;
   }
   if (on) {
+    if constexpr (REGS != 0) {
+#pragma unroll
+      for (uint32_t k = 0; k < R; ++k) {
+        if (t0 + k < t1) col[(size_t)(t0 + k) * nb] = run;
+        run += c[k];
+      }
+    } else {
 #pragma unroll 8
-    for (uint32_t t = t0; t < t1; ++t) {
-      const size_t at = (size_t)t * nb + d;
-      const uint32_t c = hist[at];
-      hist[at] = run;
-      run += c;
+      for (uint32_t t = t0; t < t1; ++t) {
+        const size_t at = (size_t)t * nb;
+        const uint32_t v = col[at];
+        col[at] = run;
+        run += v;
+      }
     }
     if (g == 0) totals[d] = all;
   }
@@ -191,16 +192,24 @@ __global__ __launch_bounds__(kScanBlock) void k_lsd_scan(uint32_t* __restrict__ 
 // nullptr stands for the identity (value = input index). Wave w of the tile
 // owns keys [w*512, w*512+512) of the tile, visited in 8 rounds of 64
 // consecutive keys, so (wave, round, lane) order is input order.
+//
+// LDS, 28 KB (five work-groups per CU): 16-bit per-wave digit counts while
+// the keys are ranked, the same bytes then holding the tile's keys and
+// values in digit order; each digit's run start in that copy (16-bit), and
+// its output start minus that run start.
 __global__ __launch_bounds__(kLsdBlock) void k_lsd_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t n,
     const BatchParams* __restrict__ bp, uint32_t pass, const uint32_t* __restrict__ offsets,
     const uint32_t* __restrict__ totals, uint32_t* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out) {
-  __shared__ uint32_t wcnt[kLsdWaves][kMaxBins];  // per wave: running count, then prefix
-  __shared__ uint32_t tile_start[kMaxBins];       // digit runs in the LDS copy of the tile
-  __shared__ uint32_t dbase[kMaxBins];            // output start of the tile's run of each digit
-  __shared__ uint32_t skey[kLsdTile];
-  __shared__ uint32_t sval[kLsdTile];
+  static_assert(kLsdWaves * kMaxBins * sizeof(uint16_t) == 2 * kLsdTile * sizeof(uint32_t),
+                "the per-wave counts and the tile copy share one array");
+  __shared__ uint4 stage[2 * kLsdTile / 4];
+  __shared__ uint16_t tile_start[kMaxBins];   // digit runs in the LDS copy of the tile
+  __shared__ uint32_t dshift[kMaxBins];       // output start of a digit's run - tile_start
+  uint16_t(*wcnt)[kMaxBins] = reinterpret_cast<uint16_t(*)[kMaxBins]>(stage);
+  uint32_t* skey = reinterpret_cast<uint32_t*>(stage);
+  uint32_t* sval = skey + kLsdTile;
   if (pass >= lsd_passes_of(bp)) return;
   const uint32_t db = bp->digit_bits, nb = 1u << db;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -212,13 +221,12 @@ __global__ __launch_bounds__(kLsdBlock) void k_lsd_scatter(
     kk[t] = i < n ? keys_in[i] : 0u;
     vv[t] = i < n ? (vals_in ? vals_in[i] : i) : 0u;
   }
-  for (uint32_t d = threadIdx.x; d < nb; d += kLsdBlock)
-#pragma unroll
-    for (int q = 0; q < kLsdWaves; ++q) wcnt[q][d] = 0;
+  for (uint32_t q = threadIdx.x; q < 2 * kLsdTile / 4; q += kLsdBlock)
+    stage[q] = make_uint4(0u, 0u, 0u, 0u);
   // where each digit's run of this tile starts in the output: the digit's
   // start (exclusive scan of the totals) plus the tiles before this one
+  constexpr int per = kMaxBins / kLsdBlock;
   {
-    constexpr int per = kMaxBins / kLsdBlock;
     uint32_t v[per], o[per];
     uint32_t local = 0;
 #pragma unroll
@@ -232,7 +240,7 @@ __global__ __launch_bounds__(kLsdBlock) void k_lsd_scatter(
 #pragma unroll
     for (int k = 0; k < per; ++k) {
       const uint32_t d = threadIdx.x * per + k;
-      if (d < nb) dbase[d] = run + o[k];
+      if (d < nb) dshift[d] = run + o[k];
       run += v[k];
     }
   }
@@ -250,43 +258,64 @@ __global__ __launch_bounds__(kLsdBlock) void k_lsd_scatter(
     uint32_t prev = 0;
     if (valid && peer == 0) {
       prev = wcnt[w][d];
-      wcnt[w][d] = prev + (uint32_t)__builtin_popcountll(m);
+      wcnt[w][d] = (uint16_t)(prev + (uint32_t)__builtin_popcountll(m));
     }
     prev = __shfl(prev, m ? (int)__builtin_ctzll(m) : 0, 64);
     rr[t] = prev + peer;
   }
   __syncthreads();
-  // per digit: prefix over waves and the tile's count
-  for (uint32_t d = threadIdx.x; d < kMaxBins; d += kLsdBlock) {
-    uint32_t run = 0;
-    if (d < nb) {
+  // per digit: the waves' counts turned into their prefix, the tile's count,
+  // and the digit's run start in the tile (at most 2048: 16 bits)
+  {
+    uint32_t c[per];
+    uint32_t local = 0;
 #pragma unroll
-      for (int q = 0; q < kLsdWaves; ++q) {
-        const uint32_t c = wcnt[q][d];
-        wcnt[q][d] = run;
-        run += c;
+    for (int k = 0; k < per; ++k) {
+      const uint32_t d = threadIdx.x * per + k;
+      uint32_t run = 0;
+      if (d < nb) {
+#pragma unroll
+        for (int q = 0; q < kLsdWaves; ++q) {
+          const uint32_t cq = wcnt[q][d];
+          wcnt[q][d] = (uint16_t)run;
+          run += cq;
+        }
       }
+      c[k] = run;
+      local += run;
     }
-    tile_start[d] = run;
+    uint32_t run = block_exclusive_scan<kLsdBlock>(local);
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+      const uint32_t d = threadIdx.x * per + k;
+      if (d < nb) {
+        tile_start[d] = (uint16_t)run;
+        dshift[d] -= run;
+      }
+      run += c[k];
+    }
   }
   __syncthreads();
-  lds_exclusive_scan<kLsdBlock>(tile_start, kMaxBins);
+  // positions in the tile copy, then the copy (over the counts)
+#pragma unroll
+  for (int t = 0; t < kLsdItems; ++t) {
+    const uint32_t d = pass_digit(kk[t], db, pass);
+    rr[t] += (uint32_t)tile_start[d] + (uint32_t)wcnt[w][d];
+  }
+  __syncthreads();
 #pragma unroll
   for (int t = 0; t < kLsdItems; ++t) {
     const uint32_t i = base + t * 64 + lane;
     if (i >= n) continue;
-    const uint32_t d = pass_digit(kk[t], db, pass);
-    const uint32_t j = tile_start[d] + wcnt[w][d] + rr[t];
-    skey[j] = kk[t];
-    sval[j] = vv[t];
+    skey[rr[t]] = kk[t];
+    sval[rr[t]] = vv[t];
   }
   __syncthreads();
   const uint32_t t0 = blockIdx.x * kLsdTile;
   const uint32_t count = min((uint32_t)kLsdTile, n - t0);
   for (uint32_t j = threadIdx.x; j < count; j += kLsdBlock) {
     const uint32_t key = skey[j];
-    const uint32_t d = pass_digit(key, db, pass);
-    const uint32_t dst = dbase[d] + (j - tile_start[d]);
+    const uint32_t dst = dshift[pass_digit(key, db, pass)] + j;
     keys_out[dst] = key;
     vals_out[dst] = sval[j];
   }
@@ -660,13 +689,24 @@ void launch_sort(clsph_state* s) {
   // (key_t, val_t) and (key_u, val_u) in turn
   const uint32_t* kin = s->key_a;
   const uint32_t* vin = nullptr;
+  // tiles per slice of the scan (64 slices): held in registers up to 32
+  const uint32_t per = (tiles + kScanBlock / 16 - 1) / (kScanBlock / 16);
+  auto scan = [&](uint32_t p) {
+    auto go = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, dim3(kMaxBins / 16), dim3(kScanBlock), 0, st, s->sort_hist, tiles,
+                         s->bp, p, s->sort_totals);
+    };
+    if (per <= 8) go(k_lsd_scan<8>);
+    else if (per <= 16) go(k_lsd_scan<16>);
+    else if (per <= 32) go(k_lsd_scan<32>);
+    else go(k_lsd_scan<0>);
+  };
   for (uint32_t p = 0; p < s->sort_passes; ++p) {
     uint32_t* kout = (p & 1) ? s->key_u : s->key_t;
     uint32_t* vout = (p & 1) ? s->val_u : s->val_t;
     hipLaunchKernelGGL(k_lsd_count, dim3(tiles), dim3(kLsdBlock), 0, st, kin, s->n, s->bp, p,
                        s->sort_hist, s->sort_runs_ctl);
-    hipLaunchKernelGGL(k_lsd_scan, dim3(kMaxBins / 16), dim3(kScanBlock), 0, st, s->sort_hist,
-                       tiles, s->bp, p, s->sort_totals);
+    scan(p);
     hipLaunchKernelGGL(k_lsd_scatter, dim3(tiles), dim3(kLsdBlock), 0, st, kin, vin, s->n, s->bp,
                        p, s->sort_hist, s->sort_totals, kout, vout);
     kin = kout;
