@@ -165,7 +165,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_advect(
     const float4* __restrict__ acc_s, const uint32_t* __restrict__ slot, Mesh mesh,
     const BlockDesc* __restrict__ blocks, DevParams* __restrict__ dps,
     float4* __restrict__ pos, float4* __restrict__ vel, float4* __restrict__ vh,
-    float* __restrict__ aabb_partial, uint32_t n) {
+    float* __restrict__ aabb_partial, uint32_t aabb_stride, uint32_t n) {
   uint32_t scene;
   uint32_t i = particle_of(blocks, dps, scene);
   bool valid = i != 0xFFFFFFFFu;
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_advect(
   // Work-group bounding box of the new positions for the next substep's grid
   // (the work-group lies in one scene), and the count of particles that
   // reached the bounce cap.
-  block_aabb_partial(outp, valid, aabb_partial);
+  block_aabb_partial(outp, valid, aabb_partial, aabb_stride);
   __shared__ uint32_t cap_red[kMaxBlock / 64];
   uint64_t cb = __builtin_amdgcn_ballot_w64(capped != 0);
   if ((threadIdx.x & 63) == 0) cap_red[threadIdx.x >> 6] = (uint32_t)__builtin_popcountll(cb);
@@ -236,7 +236,7 @@ void launch_advect(clsph_state* s) {
   Mesh m{s->face_normals, s->vertices, s->indices, s->tgrid_start, s->tgrid_tris};
   hipLaunchKernelGGL(k_advect, dim3(s->nblocks_of(kStageAdvect)), dim3(s->bs_of(kStageAdvect)), 0,
                      s->ctx->stream, s->pos_s, s->vh_s, s->acc_s, s->slot, m,
-                     s->table_of(kStageAdvect), s->dps, s->pos, s->vel, s->vh, s->aabb_partial,
+                     s->table_of(kStageAdvect), s->dps, s->pos, s->vel, s->vh, s->aabb_partial, s->aabb_stride,
                      s->n);
 }
 

@@ -363,7 +363,8 @@ int alloc_state(clsph_ctx* ctx, uint32_t n_scenes, const uint32_t* caps, bool fi
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->sort_runs_ctl, 2);
   if (rc == CLSPH_OK) HIPCK(ctx, hipMemset(st->sort_runs_ctl, 0, 2 * sizeof(uint32_t)));
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->table, (size_t)st->table_cap + 1);
-  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->aabb_partial, (size_t)(blocks64 + 1) * 6);
+  st->aabb_stride = (uint32_t)(blocks64 + 1);
+  if (rc == CLSPH_OK) rc = dalloc(ctx, &st->aabb_partial, (size_t)st->aabb_stride * 6);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->dps, n_scenes);
   if (rc == CLSPH_OK) rc = dalloc(ctx, &st->bp, 1);
   if (rc != CLSPH_OK) {

@@ -23,12 +23,12 @@ namespace clsph {
 __global__ __launch_bounds__(kMaxBlock) void k_aabb(const float4* __restrict__ pos,
                                                     const BlockDesc* __restrict__ blocks,
                                                     const DevParams* __restrict__ dps,
-                                                    float* __restrict__ partial) {
+                                                    float* __restrict__ partial, uint32_t stride) {
   uint32_t scene;
   uint32_t i = particle_of(blocks, dps, scene);
   bool valid = i != 0xFFFFFFFFu;
   float4 p = valid ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-  block_aabb_partial(p, valid, partial);
+  block_aabb_partial(p, valid, partial, stride);
 }
 
 // The batch's key total, key width and sort digit width for `base` keys in
@@ -96,6 +96,7 @@ __device__ uint32_t scene_grid(DevParams* __restrict__ dp, const float v[6], flo
 
 __global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dps,
                                                       const float* __restrict__ partial,
+                                                      uint32_t stride,
                                                       BatchParams* __restrict__ bp) {
   __shared__ float red[6][16];
   DevParams* dp = dps + blockIdx.x;
@@ -112,7 +113,7 @@ __global__ __launch_bounds__(1024) void k_grid_params(DevParams* __restrict__ dp
     for (int u = 0; u < 8; ++u) {
       const uint32_t bu = b + (uint32_t)u * blockDim.x;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) q[u][k] = bu < b1 ? partial[bu * 6 + k] : (k < 3 ? inf : -inf);
+      for (int k = 0; k < 6; ++k) q[u][k] = bu < b1 ? partial[k * stride + bu] : (k < 3 ? inf : -inf);
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -223,12 +224,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_hash(const float4* __restrict__ p
 void launch_aabb(clsph_state* s) {
   if (s->n == 0) return;
   hipLaunchKernelGGL(k_aabb, dim3(s->nblocks_of(kStageAdvect)), dim3(s->bs_of(kStageAdvect)), 0,
-                     s->ctx->stream, s->pos, s->table_of(kStageAdvect), s->dps, s->aabb_partial);
+                     s->ctx->stream, s->pos, s->table_of(kStageAdvect), s->dps, s->aabb_partial,
+                     s->aabb_stride);
 }
 
 void launch_grid_params(clsph_state* s) {
   hipLaunchKernelGGL(k_grid_params, dim3(s->n_scenes), dim3(1024), 0, s->ctx->stream, s->dps,
-                     s->aabb_partial, s->bp);
+                     s->aabb_partial, s->aabb_stride, s->bp);
 }
 
 void launch_hash(clsph_state* s) {

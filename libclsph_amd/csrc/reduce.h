@@ -14,10 +14,13 @@ __device__ inline float wave_max(float v) {
   return v;
 }
 
-// Work-group min/max of (x, y, z) into partial[blockIdx.x * 6 + k]; NaN
-// coordinates are skipped, like the reference's `<` / `>` tests
-// (sph_simulation.cpp:207-217). Every thread of the work-group must call it.
-__device__ inline void block_aabb_partial(float4 p, bool valid, float* __restrict__ partial) {
+// Work-group min/max of (x, y, z) into partial[k * stride + blockIdx.x]
+// (k = 0-2 the minima, 3-5 the maxima: one array per bound, so that the
+// reader loads consecutive words); NaN coordinates are skipped, like the
+// reference's `<` / `>` tests (sph_simulation.cpp:207-217). Every thread of
+// the work-group must call it.
+__device__ inline void block_aabb_partial(float4 p, bool valid, float* __restrict__ partial,
+                                          uint32_t stride) {
   __shared__ float red[6][kMaxBlock / 64];
   const float inf = __builtin_huge_valf();
   float v[6];
@@ -38,7 +41,7 @@ __device__ inline void block_aabb_partial(float4 p, bool valid, float* __restric
     int nw = (blockDim.x + 63) >> 6;
     float r = red[k][0];
     for (int q = 1; q < nw; ++q) r = (k < 3) ? fminf(r, red[k][q]) : fmaxf(r, red[k][q]);
-    partial[blockIdx.x * 6 + k] = r;
+    partial[k * stride + blockIdx.x] = r;
   }
 }
 

@@ -89,10 +89,12 @@ struct clsph_state {
   uint32_t* ids = nullptr;
   uint32_t* ids_tmp = nullptr;
 
-  // Per-work-group bounding boxes (6 floats each) written by the bounding-box
-  // kernel or by advection over the advection work-group table, reduced per
-  // scene by the grid-parameter kernel.
+  // Per-work-group bounding boxes written by the bounding-box kernel or by
+  // advection over the advection work-group table, reduced per scene by the
+  // grid-parameter kernel: bound k (min x, y, z, max x, y, z) of work-group b
+  // at aabb_partial[k * aabb_stride + b].
   float* aabb_partial = nullptr;
+  uint32_t aabb_stride = 0;
 
   // Work-group tables, one per block size (kBlockSizes), and the block size
   // each stage runs with. Advection and the bounding-box kernel share one.
