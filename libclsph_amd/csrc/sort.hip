@@ -389,11 +389,6 @@ __device__ __forceinline__ void cells_place(
   // and in pre[w][0..7] / pre[w][8] the octant counts and cell starts in
   // [a, start of this wave's 64 positions) (carry[] holds those counts for
   // the chunk start and is advanced past the chunk).
-  // One chunk of 256 positions from k0: the key, whether the position starts
-  // a cell, the wave's ballots of cell starts (sm) and of each octant (bq),
-  // and in pre[w][0..7] / pre[w][8] the octant counts and cell starts in
-  // [a, start of this wave's 64 positions) (carry[] holds those counts for
-  // the chunk start and is advanced past the chunk).
 #define CELLS_CHUNK(k0, end, key, valid, st, sm, bq)                                  \
   do {                                                                                \
     const uint32_t k_ = (k0) + threadIdx.x;                                          \
