@@ -162,6 +162,54 @@ def test_three_pass_state_parity(ctx, case):
     check_against_oracle(sc.particles, sc.params, g, ref)
 
 
+# Sizes whose tile counts (2,048 keys a tile, 64 slices of tiles per digit in
+# the scan) take each form of the sort's scan: at most 16 tiles a slice
+# (733 tiles), at most 32 (1,465) and more than 32 (2,198: counts loaded
+# twice). The sparse clouds' 23-bit cell ids need the third LSD pass.
+LARGE_SORT_CASES = {
+    "dam1500k_two_passes": (lambda: scenes.dam_break(1_500_000), 1 << 22),
+    "sparse3m_three_passes": (lambda: sparse_cloud(3_000_000, 17), 1 << 24),
+    "sparse4500k_three_passes": (lambda: sparse_cloud(4_500_000, 19), 1 << 24),
+}
+
+
+@pytest.mark.parametrize("case", list(LARGE_SORT_CASES))
+def test_grid_sort_table_parity_at_scale(ctx, case):
+    """Grid parameters, cell ids, the stable order and the cell table, all
+    bit-exact against the oracle's grid, hash, sort and table stages."""
+    print(f"case {case}", flush=True)
+    make, cap = LARGE_SORT_CASES[case]
+    sc = make()
+    p = sc.params.copy()
+    assert oracle.grid_params(sc.particles, p) == 0
+    hashed = oracle.locate_in_grid(sc.particles, p)
+    srt = oracle.sort_particles(hashed)
+    G = int(p["grid_cell_count"])
+    assert G <= cap
+    st = native.State(ctx, sc.n, table_cap=cap)
+    try:
+        st.set_params(sc.params, sc.terms)
+        st.scene(sc.normals, sc.vertices, sc.indices)
+        st.upload(sc.particles)
+        st.stage("grid")
+        g = st.grid()
+        st.stage("hash")
+        keys = st.keys()
+        st.stage("sort")  # also builds the cell table
+        skeys, perm = st.sorted()
+        table = st.cell_table(G)
+    finally:
+        st.close()
+    for f in ("min_point", "max_point"):
+        np.testing.assert_array_equal(g[f][:3], p[f][:3], err_msg=f)
+    for f in ("grid_size_x", "grid_size_y", "grid_size_z", "grid_cell_count"):
+        assert int(g[f]) == int(p[f]), f
+    np.testing.assert_array_equal(keys, hashed["grid_index"])
+    np.testing.assert_array_equal(skeys, srt["grid_index"])
+    np.testing.assert_array_equal(perm, np.argsort(hashed["grid_index"], kind="stable"))
+    np.testing.assert_array_equal(table, oracle.cell_table(srt, G))
+
+
 def check_against_oracle(parts, params, g, ref):
     """Compares the readbacks of stage_outputs with the oracle's stages of
     the same substep from the same particles `parts`."""
