@@ -47,6 +47,36 @@ def test_batch_equals_scenes_alone(ctx):
     b.close()
 
 
+def ragged_scene(n, seed):
+    """A dam break cut to its first n particles (n may be 0)."""
+    sc = scenes.dam_break(max(n, 64), seed=seed)
+    p = sc.params.copy()
+    p["particles_count"] = n
+    return scenes.Scene(sc.particles[:n].copy(), p, sc.terms, sc.normals, sc.vertices,
+                        sc.indices)
+
+
+def test_ragged_batch_with_an_empty_scene(ctx):
+    """Scenes of 5,000, 0, 1, 63 and 2,049 particles in one batch (an empty
+    scene, a lone particle, less than a wave, one past a sort tile): after 10
+    substeps every non-empty scene is bit-identical to itself run alone, and
+    the empty scene holds nothing and raises no error."""
+    counts = [5000, 0, 1, 63, 2049]
+    print(f"case ragged batch {counts}, 10 substeps", flush=True)
+    scs = [ragged_scene(c, 31 + i) for i, c in enumerate(counts)]
+    b = autotune.make_batch(ctx, scs)
+    b.step(10)
+    for i, sc in enumerate(scs):
+        got = b.download_scene(i)
+        assert len(got) == counts[i]
+        if counts[i] == 0:
+            continue
+        want = run_alone(ctx, sc, 10)
+        for f in FIELDS:
+            np.testing.assert_array_equal(got[f], want[f], err_msg=f"scene {i} {f}")
+    b.close()
+
+
 def test_block_sizes_do_not_change_results(ctx):
     print("case batch work-group sizes", flush=True)
     scs = small_batch()
