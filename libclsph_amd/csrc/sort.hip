@@ -31,8 +31,6 @@ namespace clsph {
 constexpr int kLsdBlock = 256;
 constexpr int kLsdItems = 8;                         // keys per thread
 constexpr int kLsdTile = kLsdBlock * kLsdItems;      // 2048 keys per tile
-constexpr int kLsdWaves = kLsdBlock / 64;
-constexpr int kLsdWaveKeys = kLsdTile / kLsdWaves;   // 512 consecutive keys per wave
 constexpr int kScanBlock = 1024;                     // 16 digits x 64 slices of the tiles
 
 constexpr int kCellBlock = 256;
@@ -189,22 +187,28 @@ __global__ __launch_bounds__(kScanBlock) void k_lsd_scan(uint32_t* __restrict__ 
 }
 
 // Stable scatter of (key, value) by the digit of pass `pass`; vals_in ==
-// nullptr stands for the identity (value = input index). Wave w of the tile
-// owns keys [w*512, w*512+512) of the tile, visited in 8 rounds of 64
+// nullptr stands for the identity (value = input index). A tile's 2048 keys
+// are spread over 8 waves, so that each wave's ranking chain is short: wave
+// w owns keys [w*256, w*256+256) of the tile, visited in 4 rounds of 64
 // consecutive keys, so (wave, round, lane) order is input order.
 //
-// LDS, 28 KB (five work-groups per CU): 16-bit per-wave digit counts while
-// the keys are ranked, the same bytes then holding the tile's keys and
-// values in digit order; each digit's run start in that copy (16-bit), and
-// its output start minus that run start.
-__global__ __launch_bounds__(kLsdBlock) void k_lsd_scatter(
+// LDS, 45 KB (three work-groups, 24 waves per CU): 16-bit per-wave digit
+// counts while the keys are ranked, the first half of the same bytes then
+// holding the tile's keys and values in digit order; each digit's run start
+// in that copy (16-bit), and its output start minus that run start.
+constexpr int kScatBlock = 512;                         // scatter threads per tile
+constexpr int kScatItems = kLsdTile / kScatBlock;       // keys per thread
+constexpr int kScatWaves = kScatBlock / 64;
+constexpr int kScatWaveKeys = kLsdTile / kScatWaves;    // consecutive keys per wave
+__global__ __launch_bounds__(kScatBlock) void k_lsd_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t n,
     const BatchParams* __restrict__ bp, uint32_t pass, const uint32_t* __restrict__ offsets,
     const uint32_t* __restrict__ totals, uint32_t* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out) {
-  static_assert(kLsdWaves * kMaxBins * sizeof(uint16_t) == 2 * kLsdTile * sizeof(uint32_t),
-                "the per-wave counts and the tile copy share one array");
-  __shared__ uint4 stage[2 * kLsdTile / 4];
+  static_assert(kScatWaves * kMaxBins * sizeof(uint16_t) >= 2 * kLsdTile * sizeof(uint32_t),
+                "the tile copy fits in the bytes of the per-wave counts");
+  constexpr int kStageQ = kScatWaves * kMaxBins * (int)sizeof(uint16_t) / 16;
+  __shared__ uint4 stage[kStageQ];
   __shared__ uint16_t tile_start[kMaxBins];   // digit runs in the LDS copy of the tile
   __shared__ uint32_t dshift[kMaxBins];       // output start of a digit's run - tile_start
   uint16_t(*wcnt)[kMaxBins] = reinterpret_cast<uint16_t(*)[kMaxBins]>(stage);
@@ -213,19 +217,19 @@ __global__ __launch_bounds__(kLsdBlock) void k_lsd_scatter(
   if (pass >= lsd_passes_of(bp)) return;
   const uint32_t db = bp->digit_bits, nb = 1u << db;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint32_t base = blockIdx.x * kLsdTile + w * kLsdWaveKeys;
-  uint32_t kk[kLsdItems], vv[kLsdItems], rr[kLsdItems];
+  const uint32_t base = blockIdx.x * kLsdTile + w * kScatWaveKeys;
+  uint32_t kk[kScatItems], vv[kScatItems], rr[kScatItems];
 #pragma unroll
-  for (int t = 0; t < kLsdItems; ++t) {
+  for (int t = 0; t < kScatItems; ++t) {
     const uint32_t i = base + t * 64 + lane;
     kk[t] = i < n ? keys_in[i] : 0u;
     vv[t] = i < n ? (vals_in ? vals_in[i] : i) : 0u;
   }
-  for (uint32_t q = threadIdx.x; q < 2 * kLsdTile / 4; q += kLsdBlock)
+  for (uint32_t q = threadIdx.x; q < kStageQ; q += kScatBlock)
     stage[q] = make_uint4(0u, 0u, 0u, 0u);
   // where each digit's run of this tile starts in the output: the digit's
   // start (exclusive scan of the totals) plus the tiles before this one
-  constexpr int per = kMaxBins / kLsdBlock;
+  constexpr int per = kMaxBins / kScatBlock;
   {
     uint32_t v[per], o[per];
     uint32_t local = 0;
@@ -236,7 +240,7 @@ __global__ __launch_bounds__(kLsdBlock) void k_lsd_scatter(
       o[k] = d < nb ? offsets[(size_t)blockIdx.x * nb + d] : 0u;
       local += v[k];
     }
-    uint32_t run = block_exclusive_scan<kLsdBlock>(local);
+    uint32_t run = block_exclusive_scan<kScatBlock>(local);
 #pragma unroll
     for (int k = 0; k < per; ++k) {
       const uint32_t d = threadIdx.x * per + k;
@@ -247,7 +251,7 @@ __global__ __launch_bounds__(kLsdBlock) void k_lsd_scatter(
   __syncthreads();
   const uint64_t lt = lanes_below(lane);
 #pragma unroll
-  for (int t = 0; t < kLsdItems; ++t) {
+  for (int t = 0; t < kScatItems; ++t) {
     const uint32_t i = base + t * 64 + lane;
     const bool valid = i < n;
     const uint32_t d = pass_digit(kk[t], db, pass);
@@ -275,7 +279,7 @@ __global__ __launch_bounds__(kLsdBlock) void k_lsd_scatter(
       uint32_t run = 0;
       if (d < nb) {
 #pragma unroll
-        for (int q = 0; q < kLsdWaves; ++q) {
+        for (int q = 0; q < kScatWaves; ++q) {
           const uint32_t cq = wcnt[q][d];
           wcnt[q][d] = (uint16_t)run;
           run += cq;
@@ -284,7 +288,7 @@ __global__ __launch_bounds__(kLsdBlock) void k_lsd_scatter(
       c[k] = run;
       local += run;
     }
-    uint32_t run = block_exclusive_scan<kLsdBlock>(local);
+    uint32_t run = block_exclusive_scan<kScatBlock>(local);
 #pragma unroll
     for (int k = 0; k < per; ++k) {
       const uint32_t d = threadIdx.x * per + k;
@@ -298,13 +302,13 @@ __global__ __launch_bounds__(kLsdBlock) void k_lsd_scatter(
   __syncthreads();
   // positions in the tile copy, then the copy (over the counts)
 #pragma unroll
-  for (int t = 0; t < kLsdItems; ++t) {
+  for (int t = 0; t < kScatItems; ++t) {
     const uint32_t d = pass_digit(kk[t], db, pass);
     rr[t] += (uint32_t)tile_start[d] + (uint32_t)wcnt[w][d];
   }
   __syncthreads();
 #pragma unroll
-  for (int t = 0; t < kLsdItems; ++t) {
+  for (int t = 0; t < kScatItems; ++t) {
     const uint32_t i = base + t * 64 + lane;
     if (i >= n) continue;
     skey[rr[t]] = kk[t];
@@ -313,7 +317,7 @@ __global__ __launch_bounds__(kLsdBlock) void k_lsd_scatter(
   __syncthreads();
   const uint32_t t0 = blockIdx.x * kLsdTile;
   const uint32_t count = min((uint32_t)kLsdTile, n - t0);
-  for (uint32_t j = threadIdx.x; j < count; j += kLsdBlock) {
+  for (uint32_t j = threadIdx.x; j < count; j += kScatBlock) {
     const uint32_t key = skey[j];
     const uint32_t dst = dshift[pass_digit(key, db, pass)] + j;
     keys_out[dst] = key;
@@ -702,7 +706,7 @@ void launch_sort(clsph_state* s) {
     hipLaunchKernelGGL(k_lsd_count, dim3(tiles), dim3(kLsdBlock), 0, st, kin, s->n, s->bp, p,
                        s->sort_hist, s->sort_runs_ctl);
     scan(p);
-    hipLaunchKernelGGL(k_lsd_scatter, dim3(tiles), dim3(kLsdBlock), 0, st, kin, vin, s->n, s->bp,
+    hipLaunchKernelGGL(k_lsd_scatter, dim3(tiles), dim3(kScatBlock), 0, st, kin, vin, s->n, s->bp,
                        p, s->sort_hist, s->sort_totals, kout, vout);
     kin = kout;
     vin = vout;
