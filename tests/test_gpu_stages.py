@@ -170,6 +170,9 @@ LARGE_SORT_CASES = {
     "dam1500k_two_passes": (lambda: scenes.dam_break(1_500_000), 1 << 22),
     "sparse3m_three_passes": (lambda: sparse_cloud(3_000_000, 17), 1 << 24),
     "sparse4500k_three_passes": (lambda: sparse_cloud(4_500_000, 19), 1 << 24),
+    # the 4M heightfield pour with its default table (4n cells): three passes
+    # captured, two run
+    "heightfield4m_two_of_three": (lambda: scenes.heightfield_pour(4_194_304), 1 << 24),
 }
 
 
