@@ -160,3 +160,56 @@ def test_terrain_contact_at_the_pour_step_against_oracle(ctx):
     g = run_gpu_stages(ctx, sc)
     check_against_oracle(sc.particles, sc.params, g, ref)
     print(f"{hits} terrain hits")
+
+
+def test_heightfield_pour_4m_density_forces_against_oracle(ctx):
+    """The 4,194,304-particle heightfield pour (the benchmark's scene, its
+    default table) through grid, hash, sort, density and forces: grid, cell
+    ids, stable order and cell table bit-exact, density within 1e-5 relative
+    and forces within 1e-4 of the largest acceleration, on every particle."""
+    from test_gpu_stages import DENSITY_RTOL, FORCE_TOL
+    sc = scenes.heightfield_pour(4_194_304)
+    print(f"case heightfield pour {sc.n} particles, {len(sc.indices) // 3} triangles", flush=True)
+    p = sc.params.copy()
+    assert oracle.grid_params(sc.particles, p) == 0
+    hashed = oracle.locate_in_grid(sc.particles, p)
+    srt = oracle.sort_particles(hashed)
+    G = int(p["grid_cell_count"])
+    table = oracle.cell_table(srt, G)
+    st = native.State(ctx, sc.n)
+    try:
+        st.set_params(sc.params, sc.terms)
+        st.scene(sc.normals, sc.vertices, sc.indices)
+        st.upload(sc.particles)
+        st.stage("grid")
+        g = st.grid()
+        st.stage("hash")
+        keys = st.keys()
+        st.stage("sort")
+        skeys, perm = st.sorted()
+        gtable = st.cell_table(G)
+        st.stage("density")
+        gd = st.download_sorted()
+        st.stage("forces")
+        gf = st.download_sorted()
+    finally:
+        st.close()
+    for f in ("min_point", "max_point"):
+        np.testing.assert_array_equal(g[f][:3], p[f][:3], err_msg=f)
+    assert int(g["grid_cell_count"]) == G
+    np.testing.assert_array_equal(keys, hashed["grid_index"])
+    np.testing.assert_array_equal(skeys, srt["grid_index"])
+    np.testing.assert_array_equal(perm, np.argsort(hashed["grid_index"], kind="stable"))
+    np.testing.assert_array_equal(gtable, table)
+    print("oracle density and forces", flush=True)
+    rd = oracle.density_pressure(srt, p, sc.terms, table)
+    assert np.max(np.abs(gd["density"] - rd["density"]) / np.abs(rd["density"])) <= DENSITY_RTOL
+    scale = max(float(np.max(np.abs(rd["pressure"]))), 1e-6)
+    assert np.max(np.abs(gd["pressure"] - rd["pressure"])) <= 1e-4 * scale
+    rf = oracle.forces(rd, p, sc.terms, table)
+    ra = rf["acceleration"][:, :3].astype(np.float64)
+    ga = gf["acceleration"][:, :3].astype(np.float64)
+    amax = np.max(np.linalg.norm(ra, axis=1))
+    err = np.max(np.linalg.norm(ga - ra, axis=1))
+    assert err <= FORCE_TOL * amax, (err, amax)
+    print(f"max |a| {amax:.3f}, max error {err:.3e}")
